@@ -1,0 +1,253 @@
+#!/usr/bin/env python
+"""Throughput of the SENSE table NUFFT forward + adjoint pair on one MI355X.
+
+Workload (one "step" = one forward + one adjoint NUFFT of the whole batch):
+  batch 4, 16 coils, 320x320 complex64 images, 640x640 grid (2x), numpoints 6,
+  table oversampling 1024; golden-angle radial trajectory of 400 spokes x 640
+  samples (256000 points); sensitivity maps from complex N(0,1) low-passed by a
+  periodic Gaussian (sigma 20 px) and normalised to unit root-sum-of-squares.
+  The adjoint is applied to the forward's output (A^H A x).
+
+Prints one JSON line (rank 0, one GPU).  `value` is pairs/s with all inputs
+resident in HBM; `roofline` is for the stage that takes the most time, from HIP
+events recorded on the stream the kernels run on; `cpu_baseline` times the CPU
+oracle (a restatement of the reference table path) on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B, C, N, K, J, L = 4, 16, 320, 640, 6, 1024
+NSPOKES, NREAD = 400, 640
+METRIC = "fwd+adj NUFFT/s, 16-coil 320² radial; % HBM peak; batched & grad vs plain"
+
+
+def radial_golden(nspokes: int, nread: int) -> torch.Tensor:
+    r = -np.pi + 2 * np.pi * np.arange(nread) / nread
+    th = np.arange(nspokes) * np.pi / ((1 + np.sqrt(5)) / 2)
+    ky = (np.sin(th)[:, None] * r[None, :]).reshape(-1)
+    kx = (np.cos(th)[:, None] * r[None, :]).reshape(-1)
+    return torch.tensor(np.stack([ky, kx]), dtype=torch.float32)
+
+
+def make_smaps(gen: torch.Generator, dev) -> torch.Tensor:
+    s = torch.complex(torch.randn(B, C, N, N, generator=gen), torch.randn(B, C, N, N, generator=gen)).to(dev)
+    f = torch.fft.fftfreq(N, device=dev)
+    # periodic Gaussian low-pass, sigma = 20 px in image space -> exp(-2 pi^2 sigma^2 f^2)
+    g1 = torch.exp(-2 * (math.pi**2) * (20.0**2) * f**2)
+    s = torch.fft.ifft2(torch.fft.fft2(s) * (g1[:, None] * g1[None, :]))
+    rss = torch.sqrt(torch.sum(torch.abs(s) ** 2, dim=1, keepdim=True))
+    return (s / rss).to(torch.complex64).contiguous()
+
+
+def footprint_cells(ktraj: np.ndarray, grid: tuple, numpoints: int) -> int:
+    """Grid cells inside some sample's J-neighbourhood (the kernel's base-offset rule)."""
+    d = len(grid)
+    kk = np.asarray(grid)
+    gam = (np.float32(1) / kk.astype(np.float32)) * np.float32(2 * np.pi)
+    tm = ktraj.astype(np.float32) / gam[:, None].astype(np.float32)
+    base = 1 + np.floor(tm - np.float32(numpoints / 2)).astype(np.int64)
+    mark = np.zeros(grid, dtype=bool)
+    for off in itertools.product(range(numpoints), repeat=d):
+        mark[tuple(np.mod(base[i] + off[i], kk[i]) for i in range(d))] = True
+    return int(mark.sum())
+
+
+def stage_bytes(F: int, M: int) -> dict:
+    P, G, BC = N * N, K * K, B * C
+    return {
+        "apod_pad": B * P * 8 + BC * P * 8 + BC * G * 8,
+        "fft": 2 * BC * G * 8,
+        "interp": BC * F * 8 + 2 * M * 4 + BC * M * 8,
+        "spread": BC * M * 8 + 2 * M * 4 + BC * G * 8,
+        "ifft": 2 * BC * G * 8,
+        "crop_deapod": BC * P * 8 + BC * P * 8 + B * P * 8,
+    }
+
+
+def time_stages(ops, x, ktraj, smaps, iters: int) -> dict:
+    """Average duration of each pipeline stage, HIP events on the current stream."""
+    from torchkbnufft_amd._nufft import fft as _fft
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    fw, ad = ops
+    tabs = fw._tables()
+    names = ["apod_pad", "fft", "interp", "spread", "ifft", "crop_deapod"]
+    acc = {n: 0.0 for n in names}
+    for _ in range(iters):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+        ev[0].record()
+        g = _fft.apod_pad(x, smaps, fw.scaling_coef, (N, N), (K, K), 1.0)
+        ev[1].record()
+        g = torch.fft.fftn(g, dim=(-2, -1), norm="backward")
+        ev[2].record()
+        kd = _interp.table_interp(g, ktraj, tabs, fw.n_shift, fw.numpoints, fw.table_oversamp, fw.offsets)
+        ev[3].record()
+        g2 = _interp.table_interp_adjoint(kd, ktraj, tabs, fw.n_shift, fw.numpoints, fw.table_oversamp, fw.offsets, (K, K))
+        ev[4].record()
+        g2 = torch.fft.ifftn(g2, dim=(-2, -1), norm="forward")
+        ev[5].record()
+        _ = _fft.crop_deapod(g2, smaps, fw.scaling_coef, (N, N), 1.0)
+        ev[6].record()
+        torch.cuda.synchronize()
+        for i, n in enumerate(names):
+            acc[n] += ev[i].elapsed_time(ev[i + 1]) * 1e3
+    return {n: acc[n] / iters for n in names}
+
+
+def cpu_baseline(x, ktraj, smaps, seconds: float) -> dict:
+    from oracle import table_path as tp
+
+    n = len(os.sched_getaffinity(0))
+    torch.set_num_threads(n)
+    plan = tp.Plan((N, N), dtype=torch.complex64)
+    xc, kc, sc = x.cpu(), ktraj.cpu(), smaps.cpu()
+    # warm-up on one batch entry
+    tp.kb_table_nufft_adjoint(plan, tp.kb_table_nufft(plan, xc[:1], kc, sc[:1]), kc, sc[:1])
+    runs, t0 = 0, time.perf_counter()
+    while True:
+        kd = tp.kb_table_nufft(plan, xc, kc, sc)
+        tp.kb_table_nufft_adjoint(plan, kd, kc, sc)
+        runs += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or runs >= 5:
+            break
+    try:
+        cpu_model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except Exception:
+        cpu_model = "unknown"
+    return {
+        "value": runs / el,
+        "unit": "pairs/s",
+        "cores": torch.get_num_threads(),
+        "kind": "port",
+        "sample": f"{runs} full pair(s) (B=4, C=16, all 256000 samples) of the oracle's complex64 "
+        f"restatement of the reference table path, torch CPU ops, {torch.get_num_threads()} intra-op / "
+        f"{torch.get_num_interop_threads()} inter-op threads, {cpu_model}",
+    }
+
+
+def fp64_oracle_error(fw, x, ktraj, smaps) -> float:
+    """Forward NUFFT vs FP64 NUDFT on a 1-coil slice and a sample subset."""
+    from oracle import nudft
+
+    sel = torch.arange(0, ktraj.shape[1], 128, device=ktraj.device)
+    y = fw(x[:1], ktraj, smaps=smaps[:1, :1])[..., sel].cpu().numpy()
+    xs = (x[:1] * smaps[:1, :1]).cpu().numpy().astype(np.complex128)
+    ref = nudft.nudft(xs, ktraj[:, sel].double().cpu().numpy(), (N // 2, N // 2))
+    return float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stage-iters", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's sharding "
+            "over GPUs is described in DESIGN.md and not run here",
+            file=sys.stderr,
+        )
+        return 2
+
+    import torchkbnufft_amd as tkbn
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    gen = torch.Generator().manual_seed(0)
+    ktraj_cpu = radial_golden(NSPOKES, NREAD)
+    M = ktraj_cpu.shape[1]
+    ktraj = ktraj_cpu.to(dev)
+    x = torch.complex(torch.randn(B, 1, N, N, generator=gen), torch.randn(B, 1, N, N, generator=gen)).to(dev)
+    smaps = make_smaps(gen, dev)
+    fw = tkbn.KbNufft(im_size=(N, N), numpoints=J, table_oversamp=L, dtype=torch.complex64).to(dev)
+    ad = tkbn.KbNufftAdjoint(im_size=(N, N), numpoints=J, table_oversamp=L, dtype=torch.complex64).to(dev)
+
+    def step():
+        kd = fw(x, ktraj, smaps=smaps)
+        return ad(kd, ktraj, smaps=smaps)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+
+    F = footprint_cells(ktraj_cpu.numpy(), (K, K), J)
+    sb = stage_bytes(F, M)
+    stages = time_stages((fw, ad), x, ktraj, smaps, args.stage_iters)
+    dom = max(stages, key=stages.get)
+    achieved = sb[dom] / (stages[dom] * 1e-6) / 1e9
+    pair_bytes = sum(sb.values())
+
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": "sense-2d-radial: batch 4 x 16 coils, 320x320 -> 640x640, 400 x 640 golden-angle "
+            "radial (256000 samples), numpoints 6, table_oversamp 1024, fwd + adj per step",
+            "batch": B,
+            "coils": C,
+            "im_size": [N, N],
+            "grid_size": [K, K],
+            "klength": M,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "pair": {
+            "algorithmic_bytes": pair_bytes,
+            "hbm_frac": pair_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "footprint_frac": F / (K * K),
+        },
+        "stages_us": {k: round(v, 2) for k, v in stages.items()},
+        "stage_bytes": sb,
+        "fp64_oracle_rel_err": fp64_oracle_error(fw, x, ktraj, smaps),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(x, ktraj, smaps, args.cpu_seconds)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,137 @@
+/*
+ * tkbn_hip.h -- C ABI of libtkbn_hip.so, the MI355X (gfx950) table-mode
+ * Kaiser-Bessel NUFFT kernels.
+ *
+ * Every entry point:
+ *   - takes device pointers to contiguous arrays (complex data interleaved
+ *     re/im, i.e. the torch.view_as_real layout; trajectories real), plain
+ *     sizes, and the caller's HIP stream (hipStream_t passed as void*, NULL =
+ *     default stream);
+ *   - enqueues work on that stream only: no host synchronisation, no device
+ *     allocation (scratch space is passed in by the caller);
+ *   - returns TKBN_OK (0) or an error code; tkbn_last_error() then returns a
+ *     thread-local message.
+ *
+ * Reference interfaces replaced (torchkbnufft, file:line):
+ *   tkbn_interp_fwd      <- _nufft/interp.py:315-403  table_interp
+ *                           (+ table_interp_one_batch :154-203, calc_coef_and_indices :89-150)
+ *   tkbn_interp_adj      <- _nufft/interp.py:587-726  table_interp_adjoint
+ *                           (+ sort_data :552-584, fork_and_accum / index_add_ :407-462)
+ *   tkbn_apod_pad        <- _nufft/fft.py:36-76       fft_and_scale (scale, zero-pad; the FFT
+ *                           itself stays torch.fft) and modules/kbnufft.py:182-183 (image*smaps)
+ *   tkbn_crop_deapod     <- _nufft/fft.py:80-118      ifft_and_scale (crop, conj scale) and
+ *                           modules/kbnufft.py:404-405 (sum_c out*conj(smaps))
+ *   tkbn_interp_domega   <- no reference counterpart: the reference's autograd Functions
+ *                           return None for the trajectory (_autograd/interp.py:130,178);
+ *                           this gives d/d omega for both of them.
+ */
+#ifndef TKBN_HIP_H
+#define TKBN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TKBN_ABI_VERSION 1
+
+#if defined(__GNUC__)
+#define TKBN_API __attribute__((visibility("default")))
+#else
+#define TKBN_API
+#endif
+
+enum tkbn_status {
+  TKBN_OK = 0,
+  TKBN_ERR_INVALID = 1,     /* bad size, null pointer, inconsistent descriptor */
+  TKBN_ERR_UNSUPPORTED = 2, /* e.g. ndim > 3, numpoints > 16 */
+  TKBN_ERR_HIP = 3,         /* HIP runtime error while enqueuing */
+  TKBN_ERR_WORKSPACE = 4    /* workspace too small */
+};
+
+enum tkbn_dtype {
+  TKBN_F32 = 0, /* complex64 data / float32 trajectory and tables */
+  TKBN_F64 = 1  /* complex128 data / float64 trajectory and tables */
+};
+
+enum tkbn_adj_mode {
+  TKBN_ADJ_ATOMIC = 0,       /* sample-parallel spread, float atomics (order varies run to run) */
+  TKBN_ADJ_DETERMINISTIC = 1 /* cell-ordered gather over a sorted plan: bitwise repeatable */
+};
+
+/*
+ * Geometry of one table interpolation (the reference's tables, n_shift,
+ * numpoints, table_oversamp and offsets buffers, plus the data shapes).
+ *   grid   : (batch, coils, K_0, ..., K_{ndim-1}) complex
+ *   kdata  : (batch, coils, klength) complex
+ *   omega  : (ndim, klength) real, or (batch, ndim, klength) when omega_batched
+ *   tables : ndim device arrays of numpoints[d]*table_oversamp[d]+1 complex entries
+ * The offsets buffer of the reference is implied: all of prod_d range(numpoints[d]),
+ * last dimension fastest (_nufft/utils.py:329).
+ */
+typedef struct tkbn_interp_desc {
+  int32_t dtype;
+  int32_t ndim;
+  int64_t grid_size[3];
+  int64_t numpoints[3];
+  int64_t table_oversamp[3];
+  double n_shift[3];
+  const void* tables[3];
+  int64_t batch;
+  int64_t coils;
+  int64_t klength;
+  int32_t omega_batched;
+  int32_t reserved;
+} tkbn_interp_desc;
+
+TKBN_API int tkbn_abi_version(void);
+TKBN_API const char* tkbn_last_error(void);
+
+/* kdata = table_interp(grid, omega): gather + Kaiser-Bessel table weights + fftshift phase. */
+TKBN_API int tkbn_interp_fwd(const tkbn_interp_desc* desc, const void* grid, const void* omega, void* kdata,
+                    void* stream);
+
+/* Bytes of device scratch tkbn_interp_adj needs for `mode` (0 when none). */
+TKBN_API int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mode);
+
+/* grid = table_interp_adjoint(kdata, omega); `grid` is fully overwritten. */
+TKBN_API int tkbn_interp_adj(const tkbn_interp_desc* desc, const void* kdata, const void* omega, void* grid,
+                    int32_t mode, void* workspace, int64_t workspace_bytes, void* stream);
+
+/*
+ * Trajectory gradient of a table interpolation:
+ *   grad_omega[d, m] = Re sum_{b,c} conj(grad_kdata[b, c, m]) * d kdata[b, c, m] / d omega[d, m]
+ * with kdata = table_interp(grid, omega).  For table_interp_adjoint's gradient pass the
+ * upstream grid gradient as `grid` and the adjoint's input samples as `grad_kdata`.
+ * grad_omega has omega's shape ((ndim, M), or (batch, ndim, M) when omega_batched; shared
+ * trajectories sum over batch and coils).
+ */
+TKBN_API int tkbn_interp_domega(const tkbn_interp_desc* desc, const void* grid, const void* grad_kdata,
+                                const void* omega, void* grad_omega, void* stream);
+
+/*
+ * grid[b, c, k] = (k < im_size in every dim) ? image[b, c or 0, k] * smaps[b, c, k]
+ *                 * scaling_coef[k] * scale : 0
+ * image has image_coils (1 or coils) coils; smaps may be NULL (then image_coils == coils).
+ * `scale` folds the FFT normalisation (1 or 1/sqrt(prod K) for norm="ortho").
+ */
+TKBN_API int tkbn_apod_pad(int32_t dtype, int32_t ndim, const int64_t* im_size, const int64_t* grid_size,
+                  int64_t batch, int64_t coils, int64_t image_coils, const void* image,
+                  const void* smaps, const void* scaling_coef, double scale, void* grid,
+                  void* stream);
+
+/*
+ * image[b, c, n] = grid[b, c, n] * conj(scaling_coef[n]) * scale          (smaps == NULL)
+ * image[b, 0, n] = sum_c grid[b, c, n] * conj(scaling_coef[n]) * scale * conj(smaps[b, c, n])
+ * n runs over im_size (the crop of the first im_size entries of each grid dim).
+ */
+TKBN_API int tkbn_crop_deapod(int32_t dtype, int32_t ndim, const int64_t* im_size, const int64_t* grid_size,
+                     int64_t batch, int64_t coils, const void* grid, const void* smaps,
+                     const void* scaling_coef, double scale, void* image, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TKBN_HIP_H */

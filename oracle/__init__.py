@@ -1,0 +1,34 @@
+"""Oracle for the table-mode Kaiser-Bessel NUFFT path -- TEST INFRASTRUCTURE ONLY.
+
+Nothing in the product (``torchkbnufft_amd``) imports, calls or links anything in
+this package.  Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
+leg of ``bench.py`` use it, and only as the checker (or as the timed CPU
+baseline), never as the thing measured or shipped.
+
+Contents
+--------
+``table_path``  CPU restatement of torchkbnufft's table-mode path in torch CPU ops,
+                following the reference op order (including its float32 index
+                arithmetic) so complex64 results match the reference's.
+                Follows ``torchkbnufft/_nufft/interp.py:88-203,315-403,406-726``,
+                ``torchkbnufft/_nufft/fft.py:9-118`` and
+                ``torchkbnufft/modules/kbnufft.py:165-228,347-410``.
+``kb``          FP64 closed forms: Kaiser-Bessel kernel, its table (with the
+                reference's linear phase), its Fourier transform (apodisation),
+                and the kernel derivative.  Independent of the reference's
+                sparse-matrix construction (``_nufft/utils.py:16-254``).
+``nudft``       FP64 direct non-uniform DFT, its adjoint and its derivative with
+                respect to the trajectory, plus an exact-KB (no table) FP64
+                interpolation operator used for finite differences.
+
+Parity pins (what checks each function; see ``tests/test_oracle.py``)
+---------------------------------------------------------------------
+* ``kb.kb_table`` / ``kb.apodization``: the reference's own ``table_i`` and
+  ``scaling_coef`` buffers written by ``tests/golden/make_golden.py``; spot values
+  (``table[J*L/2] == 1``, ``table[0] == table[J*L] == 0``); numpy ``i0``.
+* ``table_path`` forward/adjoint/NUFFT: golden vectors from the reference in
+  complex64 and float64 (``tests/golden/*.npz``); dot-product adjointness;
+  FP64 NUDFT within the table-quantisation error; dense brute force on tiny sizes.
+* ``nudft``: numpy FFT on a uniform grid (closed form special case); analytic
+  derivative vs FP64 central differences.
+"""

@@ -1,0 +1,185 @@
+"""GPU parity of the table interpolation kernels (tkbn_interp_fwd / _adj / _domega).
+
+Tolerances: complex64 results must be within 1e-5 relative L2 of the reference
+(golden vectors) and of the oracle; float64 results use the reference tests'
+``torch.allclose`` defaults (rtol 1e-5, atol 1e-8).
+"""
+import numpy as np
+import pytest
+import torch
+
+import inputs as gi
+import torchkbnufft_amd as tkbn
+from conftest import create_input_plus_noise, create_ktraj, golden, rel_l2
+from oracle import nudft
+from oracle import table_path as tp
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda")
+C64_TOL = 1e-5
+
+SHAPES = [
+    ([1, 1, 19], [1, 1, 25], True),
+    ([3, 1, 13, 2], [3, 1, 18, 2], False),
+    ([1, 1, 32, 16], [1, 1, 83], True),
+    ([5, 1, 15, 12, 2], [5, 1, 83, 2], False),
+    ([3, 2, 13, 18, 12], [3, 2, 112], True),
+    ([1, 2, 17, 19, 12, 2], [1, 2, 112, 2], False),
+]
+
+
+def _im_size(shape, is_complex):
+    return shape[2:] if is_complex else shape[2:-1]
+
+
+@pytest.mark.parametrize("case", gi.C64_CASES, ids=[c[0] for c in gi.C64_CASES])
+def test_interp_c64_matches_reference(case):
+    name, im, grid, J, L, B, C, mk, use_smaps, norm, batched = case
+    print(f"[case] interp {name}", flush=True)
+    g = golden("golden_c64.npz")
+    inp = gi.c64_inputs(case)
+    kw = dict(im_size=im, grid_size=grid, numpoints=J, table_oversamp=L, dtype=torch.complex64)
+    fw = tkbn.KbInterp(**kw).to(DEV)
+    ad = tkbn.KbInterpAdjoint(**kw).to(DEV)
+    k = inp["ktraj"].to(DEV)
+    out = fw(inp["grid_in"].to(DEV), k).cpu()
+    assert rel_l2(out, g[name + "_interp"]) < C64_TOL
+    outa = ad(inp["y_in"].to(DEV), k).cpu()
+    assert rel_l2(outa, g[name + "_interp_adj"]) < C64_TOL
+
+
+def test_interp_f64_fixtures():
+    g = golden("fixtures_f64.npz")
+    for i in range(len(gi.INTERP_FIXTURES)):
+        im_size, image, ktraj = gi.interp_fixture_inputs(i)
+        ob = tkbn.KbInterp(im_size=im_size, grid_size=im_size, dtype=torch.complex128).to(DEV)
+        out = ob(image.to(DEV), ktraj.to(DEV)).cpu()
+        assert torch.allclose(out, torch.from_numpy(g[f"interp{i}_kdata"]))
+
+
+@pytest.mark.parametrize("shape, kdata_shape, is_complex", SHAPES)
+def test_interp_adjoint_dot(shape, kdata_shape, is_complex):
+    gen = torch.Generator().manual_seed(123)
+    im_size = _im_size(shape, is_complex)
+    image = create_input_plus_noise(shape, is_complex, gen).to(DEV)
+    kdata = create_input_plus_noise(kdata_shape, is_complex, gen).to(DEV)
+    ktraj = create_ktraj(len(im_size), kdata_shape[2], gen).to(DEV)
+    fw = tkbn.KbInterp(im_size=im_size, grid_size=im_size, dtype=torch.complex128).to(DEV)
+    ad = tkbn.KbInterpAdjoint(im_size=im_size, grid_size=im_size, dtype=torch.complex128).to(DEV)
+    lhs = tkbn.inner_product(fw(image, ktraj), kdata)
+    rhs = tkbn.inner_product(image, ad(kdata, ktraj))
+    assert torch.allclose(lhs, rhs)
+
+
+@pytest.mark.parametrize("shape, kdata_shape, is_complex", SHAPES)
+def test_interp_autograd_is_adjoint(shape, kdata_shape, is_complex):
+    gen = torch.Generator().manual_seed(123)
+    im_size = _im_size(shape, is_complex)
+    image = create_input_plus_noise(shape, is_complex, gen).to(DEV).requires_grad_(True)
+    kdata = create_input_plus_noise(kdata_shape, is_complex, gen).to(DEV).requires_grad_(True)
+    ktraj = create_ktraj(len(im_size), kdata_shape[2], gen).to(DEV)
+    fw = tkbn.KbInterp(im_size=im_size, grid_size=im_size, dtype=torch.complex128).to(DEV)
+    ad = tkbn.KbInterpAdjoint(im_size=im_size, grid_size=im_size, dtype=torch.complex128).to(DEV)
+    f = fw(image, ktraj)
+    a = ad(kdata, ktraj)
+    (torch.abs(f) ** 2 / 2).sum().backward()
+    (torch.abs(a) ** 2 / 2).sum().backward()
+    assert torch.allclose(image.grad, ad(f.detach(), ktraj))
+    assert torch.allclose(kdata.grad, fw(a.detach(), ktraj))
+
+
+@pytest.mark.parametrize("shape, kdata_shape", [([1, 1, 19], [1, 1, 25]), ([1, 1, 32, 16], [1, 1, 83]), ([3, 2, 13, 18, 12], [3, 2, 112])])
+def test_interp_complex_real_match(shape, kdata_shape):
+    gen = torch.Generator().manual_seed(123)
+    im_size = shape[2:]
+    image = create_input_plus_noise(shape, True, gen).to(DEV)
+    ktraj = create_ktraj(len(im_size), kdata_shape[2], gen).to(DEV)
+    fw = tkbn.KbInterp(im_size=im_size, grid_size=im_size, dtype=torch.complex128).to(DEV)
+    a = fw(image, ktraj)
+    b = torch.view_as_complex(fw(torch.view_as_real(image), ktraj))
+    assert torch.allclose(a, b)
+
+
+@pytest.mark.parametrize(
+    "shape, kdata_shape, is_complex",
+    [([3, 1, 19], [3, 1, 25], True), ([4, 1, 32, 16], [4, 1, 83], True), ([3, 2, 13, 18, 12], [3, 2, 112], True),
+     ([2, 2, 17, 19, 12, 2], [2, 2, 112, 2], False)],
+)
+def test_interp_batched_omega_equals_loop(shape, kdata_shape, is_complex):
+    gen = torch.Generator().manual_seed(123)
+    im_size = _im_size(shape, is_complex)
+    image = create_input_plus_noise(shape, is_complex, gen).to(DEV)
+    kdata = create_input_plus_noise(kdata_shape, is_complex, gen).to(DEV)
+    ktraj = (torch.rand((shape[0], len(im_size), kdata_shape[2]), generator=gen, dtype=torch.float64) * 2 * np.pi - np.pi).to(DEV)
+    fw = tkbn.KbInterp(im_size=im_size, grid_size=im_size, dtype=torch.complex128).to(DEV)
+    ad = tkbn.KbInterpAdjoint(im_size=im_size, grid_size=im_size, dtype=torch.complex128).to(DEV)
+    loop = torch.cat([fw(i.unsqueeze(0), k) for i, k in zip(image, ktraj)])
+    assert torch.allclose(loop, fw(image, ktraj))
+    loop = torch.cat([ad(d.unsqueeze(0), k) for d, k in zip(kdata, ktraj)])
+    assert torch.allclose(loop, ad(kdata, ktraj))
+
+
+def test_c1_full_size_interp_matches_oracle():
+    """256^2 image / 512^2 grid, 402 x 512 radial spokes (the CPU-runnable plain config)."""
+    print("[case] C1 interp fwd/adj 512^2 x 205824", flush=True)
+    gen = torch.Generator().manual_seed(0)
+    ktraj = gi.radial(402, 512, golden=False)
+    plan = tp.Plan((256, 256), dtype=torch.complex64)
+    grid = gi.cplx((1, 1, 512, 512), gen)
+    y = gi.cplx((1, 1, ktraj.shape[1]), gen)
+    fw = tkbn.KbInterp(im_size=(256, 256), dtype=torch.complex64).to(DEV)
+    ad = tkbn.KbInterpAdjoint(im_size=(256, 256), dtype=torch.complex64).to(DEV)
+    assert rel_l2(fw(grid.to(DEV), ktraj.to(DEV)).cpu(), tp.kb_table_interp(plan, grid, ktraj)) < C64_TOL
+    assert rel_l2(ad(y.to(DEV), ktraj.to(DEV)).cpu(), tp.kb_table_interp_adjoint(plan, y, ktraj)) < C64_TOL
+
+
+def test_nonfinite_and_empty_inputs():
+    ob = tkbn.KbInterp(im_size=(16, 16), dtype=torch.complex64).to(DEV)
+    ad = tkbn.KbInterpAdjoint(im_size=(16, 16), dtype=torch.complex64).to(DEV)
+    grid = torch.randn(1, 2, 32, 32, dtype=torch.complex64, device=DEV)
+    om = (torch.rand(2, 10, device=DEV) * 2 - 1) * np.pi
+    om[0, 3] = float("nan")
+    om[1, 7] = float("inf")
+    out = ob(grid, om)
+    torch.cuda.synchronize()
+    bad = torch.isnan(out.real) | torch.isnan(out.imag)
+    assert bad[:, :, 3].all() and bad[:, :, 7].all()
+    assert not bad[:, :, [0, 1, 2, 4, 5, 6, 8, 9]].any()
+    empty = ob(grid, torch.zeros(2, 0, device=DEV))
+    assert empty.shape == (1, 2, 0)
+    z = ad(torch.zeros(1, 2, 0, dtype=torch.complex64, device=DEV), torch.zeros(2, 0, device=DEV))
+    assert z.shape == (1, 2, 32, 32) and torch.count_nonzero(z) == 0
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_interp_domega_matches_exact_kb_fd(ndim):
+    """d/domega of Re<g, A(omega) x>: HIP gradient vs FP64 central differences of the
+    exact-KB operator (smooth in omega; the table path itself is piecewise constant)."""
+    im = {1: (40,), 2: (24, 20), 3: (10, 12, 8)}[ndim]
+    gen = torch.Generator().manual_seed(10 + ndim)
+    grid_size = tuple(2 * v for v in im)
+    M = 300
+    x = gi.cplx((1, 2) + grid_size, gen, torch.complex128)
+    gout = gi.cplx((1, 2, M), gen, torch.complex128)
+    om = create_ktraj(ndim, M, gen)
+    fw = tkbn.KbInterp(im_size=im, dtype=torch.complex128).to(DEV)
+    omd = om.to(DEV).requires_grad_(True)
+    y = fw(x.to(DEV), omd)
+    torch.sum((y * gout.to(DEV).conj()).real).backward()
+    grad = omd.grad.cpu().numpy()
+    n_shift = [v // 2 for v in im]
+    alpha = [2.34 * 6] * ndim
+
+    def f(o):
+        yy = nudft.exact_kb_interp(x.numpy(), o, im, [6] * ndim, alpha, n_shift)
+        return np.sum((yy * gout.numpy().conj()).real, axis=(0, 1))  # per-sample
+
+    h = 1e-6
+    fd = np.zeros_like(grad)
+    for d in range(ndim):
+        e = np.zeros_like(om.numpy())
+        e[d] = h
+        fd[d] = (f(om.numpy() + e) - f(om.numpy() - e)) / (2 * h)
+    err = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
+    print(f"[domega {ndim}d] rel err vs exact-KB FD: {err:.2e}")
+    assert err < 5e-3

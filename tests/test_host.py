@@ -1,0 +1,160 @@
+"""CPU: host plan, module buffers, argument validation, and the C-ABI surface.
+
+No GPU compute happens here: library calls below only exercise the argument
+checks that run before anything is enqueued.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import inputs as gi
+import torchkbnufft_amd as tkbn
+from conftest import ROOT, golden
+from torchkbnufft_amd import _lib
+from torchkbnufft_amd._nufft import utils
+
+
+@pytest.mark.parametrize("idx", range(len(gi.PLAN_CASES)))
+def test_plan_buffers_bit_identical_to_reference(idx):
+    im, grid, J, L, w = gi.PLAN_CASES[idx]
+    p = golden("plans_c128.npz")
+    out = utils.init_fn(im, grid, J, None, L, w, 0.0, torch.complex128)
+    tables = out[0]
+    for k in range(len(im)):
+        key = f"p{idx}_table{k}"
+        if key in p.files:
+            assert np.array_equal(tables[k].numpy(), p[key])
+        else:
+            j = int(p[key + "_same_as"])
+            assert torch.equal(tables[k], tables[j])
+    sc = utils.compute_scaling_coefs(out[1].tolist(), out[2].tolist(), out[4].tolist(), out[8].tolist(), out[7].tolist())
+    d = len(im)
+    for k in range(d):
+        sl = [0] * d
+        sl[k] = slice(None)
+        assert np.array_equal(sc.numpy()[tuple(sl)], p[f"p{idx}_scaling_line{k}"])
+    if f"p{idx}_scaling_coef" in p.files:
+        assert np.array_equal(sc.numpy(), p[f"p{idx}_scaling_coef"])
+
+
+def test_module_buffers_match_reference_layout():
+    ob = tkbn.KbNufft(im_size=(12, 10), numpoints=(4, 6), dtype=torch.complex64)
+    sd = ob.state_dict()
+    assert set(sd) == {
+        "table_0", "table_1", "im_size", "grid_size", "n_shift", "numpoints", "offsets",
+        "table_oversamp", "order", "alpha", "scaling_coef",
+    }
+    assert sd["table_0"].shape == (4 * 1024 + 1,) and sd["table_1"].shape == (6 * 1024 + 1,)
+    assert sd["table_0"].dtype == torch.complex64 and sd["scaling_coef"].dtype == torch.complex64
+    assert sd["n_shift"].dtype == torch.float32 and sd["alpha"].dtype == torch.float32
+    for k in ("im_size", "grid_size", "numpoints", "offsets", "table_oversamp"):
+        assert sd[k].dtype == torch.int64
+    assert sd["offsets"].shape == (24, 2)
+    # itertools.product order: last dimension fastest
+    assert sd["offsets"][:7].tolist() == [[0, 0], [0, 1], [0, 2], [0, 3], [0, 4], [0, 5], [1, 0]]
+    assert sd["grid_size"].tolist() == [24, 20] and sd["n_shift"].tolist() == [6, 5]
+    ob2 = tkbn.KbNufft(im_size=(12, 10), numpoints=(4, 6), dtype=torch.complex64)
+    ob2.load_state_dict(sd)
+
+
+def test_module_to_maps_complex_and_real():
+    ob = tkbn.KbInterp(im_size=(8, 8), dtype=torch.complex64).to(torch.complex128)
+    assert ob.table_0.dtype == torch.complex128 and ob.n_shift.dtype == torch.float64
+    assert ob.offsets.dtype == torch.int64
+    ob = ob.to(torch.float32)
+    assert ob.table_0.dtype == torch.complex64 and ob.alpha.dtype == torch.float32
+    with pytest.raises(TypeError):
+        ob.to(torch.int32)
+
+
+def _header_functions():
+    with open(os.path.join(ROOT, "include", "tkbn_hip.h")) as f:
+        text = f.read()
+    return re.findall(r"TKBN_API\s+[\w\s\*]+?\b(tkbn_\w+)\s*\(", text)
+
+
+def test_library_exports_every_header_symbol():
+    names = _header_functions()
+    assert len(names) >= 8
+    lib = _lib.load()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.tkbn_abi_version() == _lib.ABI_VERSION
+    assert set(names) <= set(_lib._SIGNATURES)
+
+
+def _desc(**kw):
+    d = _lib.InterpDesc()
+    d.dtype = kw.get("dtype", _lib.TKBN_F32)
+    d.ndim = kw.get("ndim", 2)
+    for i in range(3):
+        d.grid_size[i] = 8
+        d.numpoints[i] = kw.get("J", 6)
+        d.table_oversamp[i] = 1024
+        d.tables[i] = 1  # never dereferenced: validation fails first or sizes are empty
+    d.batch, d.coils, d.klength = kw.get("B", 1), 1, kw.get("M", 4)
+    return d
+
+
+def test_abi_rejects_bad_descriptors_without_gpu_work():
+    lib = _lib.load()
+    assert lib.tkbn_interp_fwd(None, None, None, None, None) == _lib.TKBN_ERR_INVALID
+    assert b"NULL" in lib.tkbn_last_error()
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(ndim=4)), 1, 1, 1, None) == _lib.TKBN_ERR_UNSUPPORTED
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(J=17)), 1, 1, 1, None) == _lib.TKBN_ERR_UNSUPPORTED
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(dtype=5)), 1, 1, 1, None) == _lib.TKBN_ERR_INVALID
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(B=-1)), 1, 1, 1, None) == _lib.TKBN_ERR_INVALID
+    # empty batch is a no-op
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(B=0)), 1, 1, 1, None) == _lib.TKBN_OK
+    im = _lib.i64_array([9, 9])
+    gs = _lib.i64_array([8, 8])
+    st = lib.tkbn_apod_pad(0, 2, im, gs, 1, 1, 1, 1, None, 1, 1.0, 1, None)
+    assert st == _lib.TKBN_ERR_INVALID and b"im_size" in lib.tkbn_last_error()
+    st = lib.tkbn_apod_pad(0, 2, gs, gs, 1, 2, 1, 1, None, 1, 1.0, 1, None)
+    assert st == _lib.TKBN_ERR_INVALID  # image_coils != coils without smaps
+    assert lib.tkbn_crop_deapod(0, 0, gs, gs, 1, 1, 1, None, 1, 1.0, 1, None) == _lib.TKBN_ERR_UNSUPPORTED
+    with pytest.raises(NotImplementedError):
+        _lib.check(_lib.TKBN_ERR_UNSUPPORTED, "x")
+    with pytest.raises(ValueError):
+        _lib.check(_lib.TKBN_ERR_INVALID, "x")
+
+
+def test_python_validation_matches_reference_errors():
+    ob = tkbn.KbInterp(im_size=(8, 8))
+    image = torch.randn(1, 1, 8, 8, dtype=torch.complex64)
+    with pytest.raises(ValueError, match="omega must have 2 or 3 dimensions"):
+        ob(image, torch.zeros(1, 1, 2, 5))
+    with pytest.raises(ValueError, match="last dimension must be size 2"):
+        ob(torch.randn(1, 1, 8, 8, 3), torch.zeros(2, 5))
+    with pytest.raises(ValueError, match="batch dimension must match"):
+        ob(image, torch.zeros(3, 2, 5))
+    nu = tkbn.KbNufft(im_size=(8, 8))
+    with pytest.raises(TypeError, match="smaps"):
+        nu(image, torch.zeros(2, 5), smaps=torch.randn(1, 2, 8, 8, dtype=torch.complex128))
+    with pytest.raises(NotImplementedError):
+        nu(image, torch.zeros(2, 5), interp_mats=(torch.zeros(1), torch.zeros(1)))
+
+
+def test_cpu_tensors_are_refused_not_emulated():
+    ob = tkbn.KbInterp(im_size=(8, 8))
+    with pytest.raises(RuntimeError, match="GPU"):
+        ob(torch.randn(1, 1, 16, 16, dtype=torch.complex64), torch.zeros(2, 5))
+    adj = tkbn.KbNufftAdjoint(im_size=(8, 8))
+    with pytest.raises(RuntimeError, match="GPU"):
+        adj(torch.randn(1, 1, 5, dtype=torch.complex64), torch.zeros(2, 5))
+
+
+def test_math_helpers():
+    a = torch.randn(3, 4, dtype=torch.complex128)
+    b = torch.randn(3, 4, dtype=torch.complex128)
+    assert torch.allclose(tkbn.inner_product(a, b), torch.sum(a * b.conj()))
+    ar, br = torch.view_as_real(a), torch.view_as_real(b)
+    ip = tkbn.inner_product(ar, br)
+    assert torch.allclose(torch.complex(ip[0], ip[1]), torch.sum(a * b.conj()))
+    assert torch.allclose(torch.view_as_complex(tkbn.complex_mult(ar, br)), a * b)
+    assert torch.allclose(torch.view_as_complex(tkbn.conj_complex_mult(ar, br)), a * b.conj())
+    assert torch.allclose(tkbn.imag_exp(torch.tensor([0.5])), torch.exp(1j * torch.tensor([0.5])))

@@ -1,0 +1,179 @@
+"""CPU: pin the oracle before it is trusted as the GPU parity checker.
+
+* against the reference's own outputs (golden vectors written from the reference,
+  ``tests/golden/make_golden.py``), complex64 and float64;
+* against mathematics that is independent of the restatement: the FP64 NUDFT,
+  the closed-form (exact) Kaiser-Bessel interpolation, a dense brute-force
+  adjoint, dot tests, closed-form kernel/derivative identities.
+"""
+import numpy as np
+import pytest
+import torch
+
+import inputs as gi
+from conftest import create_input_plus_noise, create_ktraj, golden, rel_l2
+from oracle import kb, nudft
+from oracle import table_path as tp
+
+
+@pytest.mark.parametrize("case", gi.C64_CASES, ids=[c[0] for c in gi.C64_CASES])
+def test_oracle_matches_reference_c64(case):
+    name, im, grid, J, L, B, C, mk, use_smaps, norm, batched = case
+    g = golden("golden_c64.npz")
+    inp = gi.c64_inputs(case)
+    plan = tp.Plan(im, grid, J, None, L, dtype=torch.complex64)
+    k, sm = inp["ktraj"], inp["smaps"]
+    fwd_i = tp.kb_table_interp(plan, inp["grid_in"], k)
+    # interpolation reproduces the reference's float32 index arithmetic: nearly all
+    # outputs are bit-identical
+    assert (fwd_i.numpy() == g[name + "_interp"]).mean() > 0.99
+    assert rel_l2(fwd_i, g[name + "_interp"]) < 1e-7
+    assert rel_l2(tp.kb_table_interp_adjoint(plan, inp["y_in"], k), g[name + "_interp_adj"]) < 1e-7
+    assert rel_l2(tp.kb_table_nufft(plan, inp["x"], k, sm, norm), g[name + "_nufft"]) < 1e-6
+    assert rel_l2(tp.kb_table_nufft_adjoint(plan, inp["y_in"], k, sm, norm), g[name + "_nufft_adj"]) < 1e-6
+
+
+def test_oracle_matches_reference_f64_fixtures():
+    g = golden("fixtures_f64.npz")
+    for i in range(len(gi.INTERP_FIXTURES)):
+        im_size, image, ktraj = gi.interp_fixture_inputs(i)
+        plan = tp.Plan(im_size, im_size, dtype=torch.complex128)
+        out = torch.view_as_real(tp.kb_table_interp(plan, torch.view_as_complex(image), ktraj))
+        assert torch.allclose(out, torch.from_numpy(g[f"interp{i}_kdata"]))
+    for i in range(len(gi.NUFFT_FIXTURES)):
+        im_size, image, ktraj = gi.nufft_fixture_inputs(i)
+        plan = tp.Plan(im_size, dtype=torch.complex128)
+        kd = tp.kb_table_nufft(plan, torch.view_as_complex(image), ktraj)
+        assert torch.allclose(torch.view_as_real(kd), torch.from_numpy(g[f"nufft{i}_kdata"]))
+        if f"nufft{i}_adj" in g.files:
+            adj = tp.kb_table_nufft_adjoint(plan, kd, ktraj)
+            assert torch.allclose(torch.view_as_real(adj), torch.from_numpy(g[f"nufft{i}_adj"]))
+
+
+@pytest.mark.parametrize("idx", range(len(gi.PLAN_CASES)))
+def test_closed_form_kb_matches_reference_plan(idx):
+    im, grid, J, L, w = gi.PLAN_CASES[idx]
+    p = golden("plans_c128.npz")
+    d = len(im)
+    gs = gi.grid_of(im, grid)
+    Js = (J,) * d if isinstance(J, int) else J
+    for k in range(d):
+        key = f"p{idx}_table{k}"
+        t = kb.kb_table(im[k], gs[k], Js[k], L, w * Js[k])
+        if key in p.files:
+            assert np.abs(t - p[key]).max() < 1e-13
+        # spot values: centre is KB(0)*exp(0) = 1, both ends are outside |u| < J/2
+        assert abs(t[Js[k] * L // 2] - 1) < 1e-15 if (Js[k] * L) % 2 == 0 else True
+        assert t[0] == 0 and t[-1] == 0
+        mid = Js[k] * L // 2
+        if (Js[k] * L) % 2 == 0:
+            r = np.arange(1, mid)
+            assert np.allclose(np.abs(t[mid + r]), np.abs(t[mid - r]), rtol=0, atol=1e-15)
+    sc = kb.apodization(im, gs, Js, [w * j for j in Js], [0.0] * d)
+    for k in range(d):
+        line = p[f"p{idx}_scaling_line{k}"]
+        sl = [0] * d
+        sl[k] = slice(None)
+        assert np.abs(sc[tuple(sl)] - line).max() / np.abs(line).max() < 1e-12
+    if f"p{idx}_scaling_coef" in p.files:
+        full = p[f"p{idx}_scaling_coef"]
+        assert np.abs(sc - full).max() / np.abs(full).max() < 1e-12
+
+
+def test_kb_ft_matches_bessel_j_form():
+    # J sin z / (z I0(alpha)) == sqrt(2 pi)(J/2) J_{1/2}(z)/(z^{1/2} I0(alpha)) via scipy
+    from scipy import special
+
+    w = np.linspace(-0.3, 0.3, 101)
+    J, a = 6, 2.34 * 6
+    z = np.sqrt((np.pi * J * w) ** 2 - a**2 + 0j)
+    ref = np.real(np.sqrt(2 * np.pi) * (J / 2) / special.iv(0, a) * special.jv(0.5, z) / z**0.5)
+    assert np.allclose(kb.kb_ft(w, J, a), ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("im_size", [(20,), (20, 24), (8, 10, 6)])
+def test_table_nufft_approximates_nudft(im_size):
+    torch.manual_seed(0)
+    plan = tp.Plan(im_size, dtype=torch.complex128)
+    x = torch.randn((2, 1) + im_size, dtype=torch.complex128)
+    om = create_ktraj(len(im_size), 300)
+    y = tp.kb_table_nufft(plan, x, om)
+    ref = nudft.nudft(x.numpy(), om.numpy(), plan.n_shift)
+    err = rel_l2(y, ref)
+    # J=6, 2x oversampling, L=1024: table quantisation dominates (~7e-4)
+    assert 1e-5 < err < 2e-3
+    d = torch.randn(2, 1, 300, dtype=torch.complex128)
+    xa = tp.kb_table_nufft_adjoint(plan, d, om)
+    ref_a = nudft.nudft_adjoint(d.numpy(), om.numpy(), im_size, plan.n_shift)
+    assert rel_l2(xa, ref_a) < 2e-3
+
+
+@pytest.mark.parametrize("im_size,grid", [((20,), (40,)), ((12, 16), (24, 32)), ((6, 8, 5), (12, 16, 10))])
+def test_table_interp_matches_exact_kb(im_size, grid):
+    # closed-form kernel at the exact distance vs table lookup at round(u*L)/L
+    torch.manual_seed(1)
+    plan = tp.Plan(im_size, grid, dtype=torch.complex128)
+    g = torch.randn((1, 2) + grid, dtype=torch.complex128)
+    om = create_ktraj(len(im_size), 400)
+    y = tp.kb_table_interp(plan, g, om)
+    ref = nudft.exact_kb_interp(g.numpy(), om.numpy(), im_size, plan.numpoints, plan.alpha, plan.n_shift)
+    assert rel_l2(y, ref) < 2e-3
+
+
+@pytest.mark.parametrize("im_size", [(7,), (5, 6)])
+def test_adjoint_is_dense_conjugate_transpose(im_size):
+    torch.manual_seed(2)
+    plan = tp.Plan(im_size, im_size, numpoints=4, dtype=torch.complex128)
+    m = 9
+    om = create_ktraj(len(im_size), m)
+    G = int(np.prod(im_size))
+    A = torch.zeros(m, G, dtype=torch.complex128)
+    for col in range(G):
+        e = torch.zeros((1, 1, G), dtype=torch.complex128)
+        e[0, 0, col] = 1
+        A[:, col] = tp.kb_table_interp(plan, e.reshape((1, 1) + im_size), om)[0, 0]
+    AH = torch.zeros(G, m, dtype=torch.complex128)
+    for row in range(m):
+        e = torch.zeros(1, 1, m, dtype=torch.complex128)
+        e[0, 0, row] = 1
+        AH[:, row] = tp.kb_table_interp_adjoint(plan, e, om).reshape(-1)
+    assert torch.allclose(AH, A.conj().T, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("im_size,smaps", [((19,), False), ((13, 12), True), ((7, 9, 6), True)])
+def test_oracle_dot_product(im_size, smaps):
+    torch.manual_seed(3)
+    plan = tp.Plan(im_size, dtype=torch.complex128)
+    C = 3
+    x = create_input_plus_noise((2, 1 if smaps else C) + im_size, True)
+    s = create_input_plus_noise((2, C) + im_size, True) if smaps else None
+    y = create_input_plus_noise((2, C, 50), True)
+    om = create_ktraj(len(im_size), 50)
+    lhs = tp.inner_product(tp.kb_table_nufft(plan, x, om, s), y)
+    rhs = tp.inner_product(x, tp.kb_table_nufft_adjoint(plan, y, om, s))
+    assert torch.allclose(lhs, rhs)
+
+
+def test_nudft_domega_matches_fd():
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((1, 6, 7)) + 1j * rng.standard_normal((1, 6, 7))
+    om = rng.uniform(-np.pi, np.pi, (2, 5))
+    an = nudft.nudft_domega(x, om, (3, 3))
+    h = 1e-6
+    for d in range(2):
+        e = np.zeros_like(om)
+        e[d] = h
+        fd = (nudft.nudft(x, om + e, (3, 3)) - nudft.nudft(x, om - e, (3, 3))) / (2 * h)
+        assert np.allclose(an[d], fd, rtol=1e-6, atol=1e-7)
+
+
+def test_kb_derivative_matches_fd():
+    u = np.linspace(-2.9, 2.9, 57)
+    h = 1e-6
+    fd = (kb.kb_value(u + h, 6, 14.04) - kb.kb_value(u - h, 6, 14.04)) / (2 * h)
+    assert np.allclose(kb.kb_deriv(u, 6, 14.04), fd, rtol=1e-6, atol=1e-8)
+    t = kb.kb_table_deriv(16, 32, 6, 1024, 14.04)
+    v = kb.kb_table(16, 32, 6, 1024, 14.04)
+    # central difference of the table itself, away from the support edges (jumps)
+    cd = (v[3:-2] - v[1:-4]) * 1024 / 2
+    assert np.abs(cd - t[2:-3]).max() / np.abs(t).max() < 1e-4

@@ -1,0 +1,129 @@
+"""Oversampled FFT stages: fused HIP scale/pad and crop/deapodise around torch.fft.
+
+``fft_and_scale`` / ``ifft_and_scale`` keep the signatures of
+``torchkbnufft/_nufft/fft.py:36-118``.  The element-wise work (scaling, zero-pad,
+crop, conjugate scaling, the SENSE coil expansion and coil combine of
+``modules/kbnufft.py:182-183,404-405``) is one HIP pass on each side; the FFT is
+``torch.fft`` (rocFFT) called unnormalised, with the ``"ortho"`` factor folded into
+the fused pass.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Optional
+
+import torch
+from torch import Tensor
+
+from .. import _lib
+from ._host import host_values, require_gpu
+
+
+def _norm_scale(norm: Optional[str], grid_size) -> float:
+    if norm is None:
+        return 1.0
+    if norm == "ortho":
+        return 1.0 / math.sqrt(float(math.prod(int(v) for v in grid_size)))
+    raise ValueError("Only option for norm is 'ortho'.")
+
+
+def apod_pad(image: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_size, grid_size, scale: float) -> Tensor:
+    """grid[b, c] = pad_end(image[b, c or 0] * smaps[b, c] * scaling_coef) * scale."""
+    require_gpu(image, smaps, scaling_coef)
+    image = image.contiguous()
+    im = tuple(int(v) for v in im_size)
+    gs = tuple(int(v) for v in grid_size)
+    d = len(im)
+    B = image.shape[0]
+    C = smaps.shape[1] if smaps is not None else image.shape[1]
+    if tuple(image.shape[2:]) != im:
+        raise ValueError(f"image spatial shape {tuple(image.shape[2:])} != im_size {im}")
+    if smaps is not None:
+        smaps = smaps.to(image.dtype).contiguous()
+        if tuple(smaps.shape) != (B, C) + im and not (smaps.shape[0] == 1 and tuple(smaps.shape[1:]) == (C,) + im):
+            raise ValueError(f"smaps shape {tuple(smaps.shape)} does not match image {tuple(image.shape)}")
+        if smaps.shape[0] != B:
+            smaps = smaps.expand(B, *smaps.shape[1:]).contiguous()
+        if image.shape[1] not in (1, C):
+            raise ValueError("image coil dimension must be 1 or match smaps")
+    sc = scaling_coef.to(device=image.device, dtype=image.dtype).contiguous()
+    if sc.numel() != math.prod(im):
+        raise ValueError("scaling_coef does not match im_size")
+    grid = torch.empty((B, C) + gs, dtype=image.dtype, device=image.device)
+    with torch.cuda.device(image.device):
+        st = _lib.load().tkbn_apod_pad(
+            _lib.dtype_code(image),
+            d,
+            _lib.i64_array(im),
+            _lib.i64_array(gs),
+            B,
+            C,
+            image.shape[1],
+            image.data_ptr(),
+            smaps.data_ptr() if smaps is not None else None,
+            sc.data_ptr(),
+            float(scale),
+            grid.data_ptr(),
+            _lib.stream_of(image),
+        )
+    _lib.check(st, "tkbn_apod_pad")
+    return grid
+
+
+def crop_deapod(grid: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_size, scale: float) -> Tensor:
+    """image[b, c] = crop(grid[b, c]) * conj(scaling_coef) * scale; with smaps summed
+    over coils against conj(smaps) into image[b, 0]."""
+    require_gpu(grid, smaps, scaling_coef)
+    grid = grid.contiguous()
+    im = tuple(int(v) for v in im_size)
+    d = len(im)
+    gs = tuple(grid.shape[2:])
+    B, C = grid.shape[0], grid.shape[1]
+    if smaps is not None:
+        smaps = smaps.to(grid.dtype).contiguous()
+        if smaps.shape[0] != B:
+            smaps = smaps.expand(B, *smaps.shape[1:]).contiguous()
+        if tuple(smaps.shape) != (B, C) + im:
+            raise ValueError(f"smaps shape {tuple(smaps.shape)} does not match data ({B}, {C}) + {im}")
+    sc = scaling_coef.to(device=grid.device, dtype=grid.dtype).contiguous()
+    out = torch.empty((B, 1 if smaps is not None else C) + im, dtype=grid.dtype, device=grid.device)
+    with torch.cuda.device(grid.device):
+        st = _lib.load().tkbn_crop_deapod(
+            _lib.dtype_code(grid),
+            d,
+            _lib.i64_array(im),
+            _lib.i64_array(gs),
+            B,
+            C,
+            grid.data_ptr(),
+            smaps.data_ptr() if smaps is not None else None,
+            sc.data_ptr(),
+            float(scale),
+            out.data_ptr(),
+            _lib.stream_of(grid),
+        )
+    _lib.check(st, "tkbn_crop_deapod")
+    return out
+
+
+def fft_and_scale(image: Tensor, scaling_coef: Tensor, im_size: Tensor, grid_size: Tensor, norm: Optional[str] = None) -> Tensor:
+    """Oversampled FFT of ``image * scaling_coef`` (``fft.py:36-76``)."""
+    from .._autograd.fft import ApodPad
+
+    gs = host_values(grid_size)
+    im = host_values(im_size)
+    scale = _norm_scale(norm, gs)
+    grid = ApodPad.apply(image, None, scaling_coef, im, gs, scale)
+    return torch.fft.fftn(grid, dim=tuple(range(-len(gs), 0)), norm="backward")
+
+
+def ifft_and_scale(image: Tensor, scaling_coef: Tensor, im_size: Tensor, grid_size: Tensor, norm: Optional[str] = None) -> Tensor:
+    """Unnormalised (or ortho) inverse FFT, crop, conjugate scaling (``fft.py:80-118``)."""
+    from .._autograd.fft import CropDeapod
+
+    gs = host_values(grid_size)
+    im = host_values(im_size)
+    scale = _norm_scale(norm, gs)
+    x = torch.fft.ifftn(image, dim=tuple(range(-len(gs), 0)), norm="forward")
+    return CropDeapod.apply(x, None, scaling_coef, im, scale)

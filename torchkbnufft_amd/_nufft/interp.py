@@ -1,0 +1,196 @@
+"""Table interpolation backends on the HIP library.
+
+``table_interp`` / ``table_interp_adjoint`` keep the signatures and errors of
+``torchkbnufft/_nufft/interp.py:315-403`` and ``:587-726``; the work is one HIP
+launch each (``tkbn_interp_fwd`` / ``tkbn_interp_adj``) instead of J^d Python-level
+iterations of ATen gathers and ``index_add_`` calls.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import torch
+from torch import Tensor
+
+from .. import _lib
+from ._host import host_values, real_dtype_of, require_gpu
+
+# adjoint mode used when the caller does not ask for one
+_DEFAULT_ADJ_MODE = _lib.TKBN_ADJ_ATOMIC
+
+
+def set_deterministic(flag: bool) -> None:
+    """Select the fixed-order (bitwise repeatable) adjoint for subsequent calls."""
+    global _DEFAULT_ADJ_MODE
+    _DEFAULT_ADJ_MODE = _lib.TKBN_ADJ_DETERMINISTIC if flag else _lib.TKBN_ADJ_ATOMIC
+
+
+def deterministic_enabled() -> bool:
+    return _DEFAULT_ADJ_MODE == _lib.TKBN_ADJ_DETERMINISTIC
+
+
+def _check_omega(omega: Tensor, batch: int, what: str) -> Tensor:
+    if omega.ndim not in (2, 3):
+        raise ValueError("omega must have 2 or 3 dimensions.")
+    if omega.ndim == 3 and omega.shape[0] == 1:
+        omega = omega[0]  # broadcast a single trajectory
+    if omega.ndim == 3 and omega.shape[0] != batch:
+        raise ValueError(f"If omega has batch dim, omega batch dimension must match {what}.")
+    return omega
+
+
+def make_desc(
+    data: Tensor,
+    omega: Tensor,
+    tables: List[Tensor],
+    n_shift,
+    numpoints,
+    table_oversamp,
+    grid_size: Sequence[int],
+    batch: int,
+    coils: int,
+) -> _lib.InterpDesc:
+    d = len(grid_size)
+    if omega.shape[-2] != d:
+        raise ValueError(f"omega has {omega.shape[-2]} rows but the grid has {d} dimensions")
+    if len(tables) != d:
+        raise ValueError(f"{len(tables)} tables for a {d}-dimensional grid")
+    jj = host_values(numpoints)
+    ll = host_values(table_oversamp)
+    ss = host_values(n_shift)
+    desc = _lib.InterpDesc()
+    desc.dtype = _lib.dtype_code(data)
+    desc.ndim = d
+    for i in range(d):
+        desc.grid_size[i] = int(grid_size[i])
+        desc.numpoints[i] = int(jj[i])
+        desc.table_oversamp[i] = int(ll[i])
+        desc.n_shift[i] = float(ss[i])
+        t = tables[i]
+        if t.dtype != data.dtype or t.device != data.device:
+            raise TypeError(f"table {i} is {t.dtype} on {t.device}; data is {data.dtype} on {data.device}")
+        if t.numel() != int(jj[i]) * int(ll[i]) + 1:
+            raise ValueError(f"table {i} has {t.numel()} entries, expected numpoints*table_oversamp+1")
+        desc.tables[i] = t.data_ptr()
+    desc.batch = batch
+    desc.coils = coils
+    desc.klength = omega.shape[-1]
+    desc.omega_batched = 1 if omega.ndim == 3 else 0
+    return desc
+
+
+def _prep_tables(tables: List[Tensor], like: Tensor) -> List[Tensor]:
+    return [t.to(device=like.device, dtype=like.dtype).contiguous() for t in tables]
+
+
+def _prep_omega(omega: Tensor, like: Tensor) -> Tensor:
+    # the trajectory is evaluated in the data's real precision (tm, rint, floor)
+    return omega.to(device=like.device, dtype=real_dtype_of(like)).contiguous()
+
+
+def table_interp(
+    image: Tensor,
+    omega: Tensor,
+    tables: List[Tensor],
+    n_shift: Tensor,
+    numpoints: Tensor,
+    table_oversamp: Tensor,
+    offsets: Tensor,
+    min_kspace_per_fork: int = 1024,
+) -> Tensor:
+    """Interpolate gridded ``image`` (B, C, *K) to ``omega``; returns (B, C, M)."""
+    omega = _check_omega(omega, image.shape[0], "image")
+    require_gpu(image, omega)
+    image = image.contiguous()
+    tables = _prep_tables(tables, image)
+    omega = _prep_omega(omega, image)
+    grid_size = tuple(image.shape[2:])
+    B, C = image.shape[0], image.shape[1]
+    desc = make_desc(image, omega, tables, n_shift, numpoints, table_oversamp, grid_size, B, C)
+    kdata = torch.empty(B, C, omega.shape[-1], dtype=image.dtype, device=image.device)
+    with torch.cuda.device(image.device):
+        st = _lib.load().tkbn_interp_fwd(
+            ctypes.byref(desc), image.data_ptr(), omega.data_ptr(), kdata.data_ptr(), _lib.stream_of(image)
+        )
+    _lib.check(st, "tkbn_interp_fwd")
+    return kdata
+
+
+def table_interp_adjoint(
+    data: Tensor,
+    omega: Tensor,
+    tables: List[Tensor],
+    n_shift: Tensor,
+    numpoints: Tensor,
+    table_oversamp: Tensor,
+    offsets: Tensor,
+    grid_size,
+    deterministic: Optional[bool] = None,
+) -> Tensor:
+    """Spread off-grid ``data`` (B, C, M) onto a grid of ``grid_size``; returns (B, C, *K)."""
+    omega = _check_omega(omega, data.shape[0], "data")
+    require_gpu(data, omega)
+    data = data.contiguous()
+    tables = _prep_tables(tables, data)
+    omega = _prep_omega(omega, data)
+    gs = tuple(int(v) for v in host_values(grid_size))
+    B, C = data.shape[0], data.shape[1]
+    if data.shape[-1] != omega.shape[-1]:
+        raise ValueError(f"data has {data.shape[-1]} samples, omega has {omega.shape[-1]}")
+    desc = make_desc(data, omega, tables, n_shift, numpoints, table_oversamp, gs, B, C)
+    mode = _DEFAULT_ADJ_MODE if deterministic is None else (
+        _lib.TKBN_ADJ_DETERMINISTIC if deterministic else _lib.TKBN_ADJ_ATOMIC
+    )
+    lib = _lib.load()
+    grid = torch.empty((B, C) + gs, dtype=data.dtype, device=data.device)
+    ws_bytes = lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(desc), mode)
+    if ws_bytes < 0:
+        _lib.check(_lib.TKBN_ERR_INVALID, "tkbn_interp_adj_workspace_bytes")
+    ws = torch.empty(max(int(ws_bytes), 1), dtype=torch.uint8, device=data.device)
+    with torch.cuda.device(data.device):
+        st = lib.tkbn_interp_adj(
+            ctypes.byref(desc),
+            data.data_ptr(),
+            omega.data_ptr(),
+            grid.data_ptr(),
+            mode,
+            ws.data_ptr(),
+            int(ws_bytes),
+            _lib.stream_of(data),
+        )
+    _lib.check(st, "tkbn_interp_adj")
+    return grid
+
+
+def table_interp_domega(
+    grid: Tensor,
+    grad_kdata: Tensor,
+    omega: Tensor,
+    tables: List[Tensor],
+    n_shift: Tensor,
+    numpoints: Tensor,
+    table_oversamp: Tensor,
+) -> Tensor:
+    """Gradient of ``Re <grad_kdata, table_interp(grid, omega)>`` with respect to ``omega``.
+
+    No reference counterpart (the reference returns ``None`` for the trajectory).
+    Returns a tensor shaped and typed like ``omega``.
+    """
+    orig = omega
+    om = _check_omega(omega, grid.shape[0], "image")
+    require_gpu(grid, grad_kdata, om)
+    grid = grid.contiguous()
+    grad_kdata = grad_kdata.to(grid.dtype).contiguous()
+    tables = _prep_tables(tables, grid)
+    om = _prep_omega(om, grid)
+    B, C = grid.shape[0], grid.shape[1]
+    desc = make_desc(grid, om, tables, n_shift, numpoints, table_oversamp, tuple(grid.shape[2:]), B, C)
+    out = torch.empty(om.shape, dtype=om.dtype, device=om.device)
+    with torch.cuda.device(grid.device):
+        st = _lib.load().tkbn_interp_domega(
+            ctypes.byref(desc), grid.data_ptr(), grad_kdata.data_ptr(), om.data_ptr(), out.data_ptr(),
+            _lib.stream_of(grid),
+        )
+    _lib.check(st, "tkbn_interp_domega")
+    return out.reshape(orig.shape).to(orig.dtype)

@@ -1,0 +1,98 @@
+// Shared definitions for the table-mode Kaiser-Bessel NUFFT kernels (gfx950).
+//
+// Conventions of every entry point (declared in include/tkbn_hip.h):
+//   * pointers are device pointers to contiguous arrays; complex data is
+//     interleaved (re, im) float2 / double2, trajectories are real;
+//   * kernels are enqueued on the caller's stream; nothing synchronises the
+//     host and nothing allocates device memory;
+//   * the return value is a status code (TKBN_OK == 0); tkbn_last_error()
+//     returns a thread-local message for the last failure on this thread.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/tkbn_hip.h"
+
+namespace tkbn {
+
+void set_error(const char* fmt, ...);
+
+#define TKBN_REQUIRE(cond, code, ...)     \
+  do {                                    \
+    if (!(cond)) {                        \
+      ::tkbn::set_error(__VA_ARGS__);     \
+      return (code);                      \
+    }                                     \
+  } while (0)
+
+#define TKBN_HIP_CHECK(expr)                                                        \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess) {                                                         \
+      ::tkbn::set_error("%s failed: %s", #expr, hipGetErrorString(_e));             \
+      return TKBN_ERR_HIP;                                                          \
+    }                                                                               \
+  } while (0)
+
+// Checks the launch that was just enqueued (configuration errors only; no sync).
+#define TKBN_LAUNCH_CHECK(name)                                                     \
+  do {                                                                              \
+    hipError_t _e = hipGetLastError();                                              \
+    if (_e != hipSuccess) {                                                         \
+      ::tkbn::set_error("launch of %s failed: %s", name, hipGetErrorString(_e));    \
+      return TKBN_ERR_HIP;                                                          \
+    }                                                                               \
+  } while (0)
+
+template <typename T>
+struct cplx_t;
+template <>
+struct cplx_t<float> {
+  using type = float2;
+};
+template <>
+struct cplx_t<double> {
+  using type = double2;
+};
+template <typename T>
+using cplx = typename cplx_t<T>::type;
+
+template <typename T>
+__host__ __device__ inline cplx<T> make_c(T re, T im) {
+  cplx<T> r;
+  r.x = re;
+  r.y = im;
+  return r;
+}
+
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+// a * conj(b)
+template <typename C>
+__device__ __forceinline__ C cmul_conj(C a, C b) {
+  C r;
+  r.x = a.x * b.x + a.y * b.y;
+  r.y = a.y * b.x - a.x * b.y;
+  return r;
+}
+
+// acc += a * b
+template <typename C>
+__device__ __forceinline__ void cmac(C& acc, C a, C b) {
+  acc.x += a.x * b.x - a.y * b.y;
+  acc.y += a.x * b.y + a.y * b.x;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace tkbn
