@@ -183,3 +183,41 @@ def test_interp_domega_matches_exact_kb_fd(ndim):
     err = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
     print(f"[domega {ndim}d] rel err vs exact-KB FD: {err:.2e}")
     assert err < 5e-3
+
+
+@pytest.mark.parametrize("im_size,grid,J", [((33, 35), (65, 70), 6), ((4, 4), (4, 4), 6), ((40,), (77,), 5), ((9, 10, 7), (17, 21, 15), (4, 6, 5))])
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_adjoint_awkward_grids_match_oracle(im_size, grid, J, deterministic):
+    """Grid sizes that are not tile multiples, and grids smaller than the kernel width."""
+    gen = torch.Generator().manual_seed(21)
+    d = len(im_size)
+    ad = tkbn.KbInterpAdjoint(im_size=im_size, grid_size=grid, numpoints=J, dtype=torch.complex64).to(DEV)
+    plan = tp.Plan(im_size, grid, J, dtype=torch.complex64)
+    M = 3000
+    om = create_ktraj(d, M, gen, torch.float32)
+    y = gi.cplx((2, 3, M), gen)
+    tkbn.set_deterministic(deterministic)
+    try:
+        out = ad(y.to(DEV), om.to(DEV)).cpu()
+    finally:
+        tkbn.set_deterministic(False)
+    assert rel_l2(out, tp.kb_table_interp_adjoint(plan, y, om)) < C64_TOL
+
+
+def test_adjoint_deterministic_is_bitwise_repeatable():
+    print("[case] deterministic adjoint, 320^2 -> 640^2 radial, 8 coils", flush=True)
+    gen = torch.Generator().manual_seed(22)
+    ktraj = gi.radial(400, 640, golden=True).to(DEV)
+    y = gi.cplx((1, 8, ktraj.shape[1]), gen).to(DEV)
+    ad = tkbn.KbInterpAdjoint(im_size=(320, 320), dtype=torch.complex64).to(DEV)
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    args = (ktraj, ad._tables(), ad.n_shift, ad.numpoints, ad.table_oversamp, ad.offsets, (640, 640))
+    outs = [_interp.table_interp_adjoint(y, *args, deterministic=True) for _ in range(3)]
+    fast = _interp.table_interp_adjoint(y, *args, deterministic=False)
+    torch.cuda.synchronize()
+    assert all(torch.equal(outs[0], o) for o in outs[1:])
+    assert rel_l2(fast, outs[0]) < 1e-6
+    plan = tp.Plan((320, 320), dtype=torch.complex64)
+    ref = tp.kb_table_interp_adjoint(plan, y[:, :2].cpu(), ktraj.cpu())
+    assert rel_l2(outs[0][:, :2].cpu(), ref) < C64_TOL

@@ -95,4 +95,22 @@ __device__ __forceinline__ void cmac(C& acc, C a, C b) {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Zero-fill n elements (a kernel, not hipMemsetAsync, so callers stay safe to
+// capture in a HIP graph).
+template <typename E>
+__global__ void __launch_bounds__(256) zero_fill_kernel(E* __restrict__ p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = E{};
+}
+
+template <typename E>
+inline int launch_zero_fill(E* p, int64_t n, hipStream_t s) {
+  if (n == 0) return TKBN_OK;
+  const int64_t blocks = ceil_div(n, 256) < 8192 ? ceil_div(n, 256) : 8192;
+  zero_fill_kernel<E><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(p, n);
+  TKBN_LAUNCH_CHECK("zero_fill_kernel");
+  return TKBN_OK;
+}
+
+
 }  // namespace tkbn
