@@ -67,41 +67,50 @@ def footprint_cells(ktraj: np.ndarray, grid: tuple, numpoints: int) -> int:
 
 
 def stage_bytes(F: int, M: int) -> dict:
+    """Algorithmic bytes per stage (the FFT stages include the model's FFT read+write)."""
     P, G, BC = N * N, K * K, B * C
     return {
-        "apod_pad": B * P * 8 + BC * P * 8 + BC * G * 8,
-        "fft": 2 * BC * G * 8,
+        "fft_fwd": B * P * 8 + BC * P * 8 + BC * G * 8 + 2 * BC * G * 8,
         "interp": BC * F * 8 + 2 * M * 4 + BC * M * 8,
         "spread": BC * M * 8 + 2 * M * 4 + BC * G * 8,
-        "ifft": 2 * BC * G * 8,
-        "crop_deapod": BC * P * 8 + BC * P * 8 + B * P * 8,
+        "fft_adj": 2 * BC * G * 8 + BC * P * 8 + BC * P * 8 + B * P * 8,
     }
 
 
 def time_stages(ops, x, ktraj, smaps, iters: int) -> dict:
-    """Average duration of each pipeline stage, HIP events on the current stream."""
+    """Average duration of each pipeline stage (the same calls the modules make),
+    HIP events on the current stream, which is the stream the kernels run on."""
     from torchkbnufft_amd._nufft import fft as _fft
     from torchkbnufft_amd._nufft import interp as _interp
 
     fw, ad = ops
     tabs = fw._tables()
-    names = ["apod_pad", "fft", "interp", "spread", "ifft", "crop_deapod"]
+    fused = _fft.fused_supported(x.dtype, (N, N), (K, K))
+    names = ["fft_fwd", "interp", "spread", "fft_adj"]
     acc = {n: 0.0 for n in names}
-    for _ in range(iters):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+
+    def run(ev):
         ev[0].record()
-        g = _fft.apod_pad(x, smaps, fw.scaling_coef, (N, N), (K, K), 1.0)
+        if fused:
+            g = _fft.fft_grid_fwd(x, smaps, fw.scaling_coef, (N, N), (K, K), 1.0)
+        else:
+            g = torch.fft.fftn(_fft.apod_pad(x, smaps, fw.scaling_coef, (N, N), (K, K), 1.0), dim=(-2, -1))
         ev[1].record()
-        g = torch.fft.fftn(g, dim=(-2, -1), norm="backward")
-        ev[2].record()
         kd = _interp.table_interp(g, ktraj, tabs, fw.n_shift, fw.numpoints, fw.table_oversamp, fw.offsets)
-        ev[3].record()
+        ev[2].record()
         g2 = _interp.table_interp_adjoint(kd, ktraj, tabs, fw.n_shift, fw.numpoints, fw.table_oversamp, fw.offsets, (K, K))
+        ev[3].record()
+        if fused:
+            _fft.fft_grid_adj(g2, smaps, fw.scaling_coef, (N, N), 1.0)
+        else:
+            _fft.crop_deapod(torch.fft.ifftn(g2, dim=(-2, -1), norm="forward"), smaps, fw.scaling_coef, (N, N), 1.0)
         ev[4].record()
-        g2 = torch.fft.ifftn(g2, dim=(-2, -1), norm="forward")
-        ev[5].record()
-        _ = _fft.crop_deapod(g2, smaps, fw.scaling_coef, (N, N), 1.0)
-        ev[6].record()
+
+    run([torch.cuda.Event(enable_timing=True) for _ in range(5)])  # warm (plans, caches)
+    torch.cuda.synchronize()
+    for _ in range(iters):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        run(ev)
         torch.cuda.synchronize()
         for i, n in enumerate(names):
             acc[n] += ev[i].elapsed_time(ev[i + 1]) * 1e3

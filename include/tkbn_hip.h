@@ -21,6 +21,10 @@
  *                           itself stays torch.fft) and modules/kbnufft.py:182-183 (image*smaps)
  *   tkbn_crop_deapod     <- _nufft/fft.py:80-118      ifft_and_scale (crop, conj scale) and
  *                           modules/kbnufft.py:404-405 (sum_c out*conj(smaps))
+ *   tkbn_fft_grid_fwd    <- _nufft/fft.py:36-76 fft_and_scale including its FFT (fft.py:9-14,
+ *                           norm None; "ortho" via `scale`) + modules/kbnufft.py:182-183
+ *   tkbn_fft_grid_adj    <- _nufft/fft.py:80-118 ifft_and_scale including its inverse FFT
+ *                           (fft.py:17-22, norm "forward"/"ortho") + modules/kbnufft.py:404-405
  *   tkbn_interp_domega   <- no reference counterpart: the reference's autograd Functions
  *                           return None for the trajectory (_autograd/interp.py:130,178);
  *                           this gives d/d omega for both of them.
@@ -34,7 +38,7 @@
 extern "C" {
 #endif
 
-#define TKBN_ABI_VERSION 1
+#define TKBN_ABI_VERSION 2
 
 #if defined(__GNUC__)
 #define TKBN_API __attribute__((visibility("default")))
@@ -88,9 +92,16 @@ typedef struct tkbn_interp_desc {
 TKBN_API int tkbn_abi_version(void);
 TKBN_API const char* tkbn_last_error(void);
 
-/* kdata = table_interp(grid, omega): gather + Kaiser-Bessel table weights + fftshift phase. */
+/*
+ * kdata = table_interp(grid, omega): gather + Kaiser-Bessel table weights + fftshift phase.
+ * With workspace (>= tkbn_interp_fwd_workspace_bytes, 16-byte aligned) the samples are
+ * binned into grid tiles that are staged through LDS; with workspace == NULL each
+ * sample gathers straight from HBM (fewer launches, for small problems).
+ * tkbn_interp_fwd_workspace_bytes returns 0 when the binned path is unavailable.
+ */
+TKBN_API int64_t tkbn_interp_fwd_workspace_bytes(const tkbn_interp_desc* desc);
 TKBN_API int tkbn_interp_fwd(const tkbn_interp_desc* desc, const void* grid, const void* omega, void* kdata,
-                    void* stream);
+                             void* workspace, int64_t workspace_bytes, void* stream);
 
 /* Bytes of device scratch tkbn_interp_adj needs for `mode` (0 when none). */
 TKBN_API int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mode);
@@ -129,6 +140,28 @@ TKBN_API int tkbn_apod_pad(int32_t dtype, int32_t ndim, const int64_t* im_size, 
 TKBN_API int tkbn_crop_deapod(int32_t dtype, int32_t ndim, const int64_t* im_size, const int64_t* grid_size,
                      int64_t batch, int64_t coils, const void* grid, const void* smaps,
                      const void* scaling_coef, double scale, void* image, void* stream);
+
+/*
+ * Fused oversampled FFT (complex64; grid sizes with prime factors 2, 3, 5 only, each
+ * <= 2048, last image dimension <= 1024).  tkbn_fft_supported returns TKBN_OK or
+ * TKBN_ERR_UNSUPPORTED for a geometry; the intermediates need
+ * tkbn_fft_workspace_bytes of caller-provided device memory.
+ *   fwd: grid[b, c] = FFT( pad_end( image[b, c or 0] * smaps[b, c] * scaling_coef ) ) * scale
+ *   adj: image[b, 0] = sum_c crop( IFFT(grid[b, c]) ) * conj(scaling_coef) * conj(smaps[b, c]) * scale
+ *        (smaps == NULL: image[b, c] per coil)
+ * FFT is unnormalised (exp(-2 pi i nk/K)), IFFT unnormalised (exp(+2 pi i nk/K)).
+ */
+TKBN_API int tkbn_fft_supported(int32_t ndim, const int64_t* im_size, const int64_t* grid_size);
+TKBN_API int64_t tkbn_fft_workspace_bytes(int32_t ndim, const int64_t* im_size, const int64_t* grid_size,
+                                          int64_t batch, int64_t coils);
+TKBN_API int tkbn_fft_grid_fwd(int32_t dtype, int32_t ndim, const int64_t* im_size, const int64_t* grid_size,
+                               int64_t batch, int64_t coils, int64_t image_coils, const void* image,
+                               const void* smaps, const void* scaling_coef, double scale, void* grid,
+                               void* workspace, int64_t workspace_bytes, void* stream);
+TKBN_API int tkbn_fft_grid_adj(int32_t dtype, int32_t ndim, const int64_t* im_size, const int64_t* grid_size,
+                               int64_t batch, int64_t coils, const void* grid, const void* smaps,
+                               const void* scaling_coef, double scale, void* image, void* workspace,
+                               int64_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -221,3 +221,26 @@ def test_adjoint_deterministic_is_bitwise_repeatable():
     plan = tp.Plan((320, 320), dtype=torch.complex64)
     ref = tp.kb_table_interp_adjoint(plan, y[:, :2].cpu(), ktraj.cpu())
     assert rel_l2(outs[0][:, :2].cpu(), ref) < C64_TOL
+
+
+@pytest.mark.parametrize("grid,rows,M", [((640, 640), 64, 40000), ((65, 70), 40, 6000), ((9, 11), 96, 3000), ((160, 160), 33, 20000)])
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_multicoil_adjoint_matches_oracle(grid, rows, M, deterministic):
+    """Many coil rows sharing one trajectory (the lanes-per-row spread)."""
+    print(f"[case] multicoil adjoint grid={grid} rows={rows} M={M} det={deterministic}", flush=True)
+    gen = torch.Generator().manual_seed(31)
+    im = tuple(max(1, g // 2) for g in grid)
+    om = create_ktraj(2, M, gen, torch.float32)
+    y = gi.cplx((1, rows, M), gen)
+    ad = tkbn.KbInterpAdjoint(im_size=im, grid_size=grid, dtype=torch.complex64).to(DEV)
+    plan = tp.Plan(im, grid, dtype=torch.complex64)
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    args = (om.to(DEV), ad._tables(), ad.n_shift, ad.numpoints, ad.table_oversamp, ad.offsets, grid)
+    out = _interp.table_interp_adjoint(y.to(DEV), *args, deterministic=deterministic)
+    if deterministic:
+        again = _interp.table_interp_adjoint(y.to(DEV), *args, deterministic=True)
+        assert torch.equal(out, again)
+    sel = [0, rows // 2, rows - 1]
+    ref = tp.kb_table_interp_adjoint(plan, y[:, sel], om)
+    assert rel_l2(out[:, sel].cpu(), ref) < C64_TOL

@@ -102,14 +102,14 @@ def _desc(**kw):
 
 def test_abi_rejects_bad_descriptors_without_gpu_work():
     lib = _lib.load()
-    assert lib.tkbn_interp_fwd(None, None, None, None, None) == _lib.TKBN_ERR_INVALID
+    assert lib.tkbn_interp_fwd(None, None, None, None, None, 0, None) == _lib.TKBN_ERR_INVALID
     assert b"NULL" in lib.tkbn_last_error()
-    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(ndim=4)), 1, 1, 1, None) == _lib.TKBN_ERR_UNSUPPORTED
-    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(J=17)), 1, 1, 1, None) == _lib.TKBN_ERR_UNSUPPORTED
-    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(dtype=5)), 1, 1, 1, None) == _lib.TKBN_ERR_INVALID
-    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(B=-1)), 1, 1, 1, None) == _lib.TKBN_ERR_INVALID
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(ndim=4)), 1, 1, 1, None, 0, None) == _lib.TKBN_ERR_UNSUPPORTED
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(J=17)), 1, 1, 1, None, 0, None) == _lib.TKBN_ERR_UNSUPPORTED
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(dtype=5)), 1, 1, 1, None, 0, None) == _lib.TKBN_ERR_INVALID
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(B=-1)), 1, 1, 1, None, 0, None) == _lib.TKBN_ERR_INVALID
     # empty batch is a no-op
-    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(B=0)), 1, 1, 1, None) == _lib.TKBN_OK
+    assert lib.tkbn_interp_fwd(ctypes.byref(_desc(B=0)), 1, 1, 1, None, 0, None) == _lib.TKBN_OK
     im = _lib.i64_array([9, 9])
     gs = _lib.i64_array([8, 8])
     st = lib.tkbn_apod_pad(0, 2, im, gs, 1, 1, 1, 1, None, 1, 1.0, 1, None)

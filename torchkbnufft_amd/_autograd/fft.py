@@ -70,3 +70,66 @@ class CropDeapod(Function):
             per_coil = crop_deapod(grid, None, scaling_coef, ctx.im_size, ctx.scale)
             grad_smaps = _sum_to(per_coil * grad.conj(), ctx.smaps_shape)
         return grad_grid, grad_smaps, None, None, None
+
+
+class GridForward(Function):
+    """image (B, 1|C, *N) [, smaps] -> FFT(pad(image * smaps * scaling)) * scale, fused."""
+
+    @staticmethod
+    def forward(ctx, image, smaps, scaling_coef, im_size, grid_size, scale):
+        from .._nufft.fft import fft_grid_fwd
+
+        ctx.im_size = tuple(im_size)
+        ctx.scale = scale
+        ctx.image_coils = image.shape[1]
+        ctx.smaps_shape = None if smaps is None else tuple(smaps.shape)
+        ctx.save_for_backward(image if (smaps is not None and smaps.requires_grad) else None, smaps, scaling_coef)
+        return fft_grid_fwd(image, smaps, scaling_coef, im_size, grid_size, scale)
+
+    @staticmethod
+    def backward(ctx, grad):
+        from .._nufft.fft import fft_grid_adj
+
+        image, smaps, scaling_coef = ctx.saved_tensors
+        grad_image = grad_smaps = per_coil = None
+        if smaps is None:
+            if ctx.needs_input_grad[0]:
+                grad_image = fft_grid_adj(grad, None, scaling_coef, ctx.im_size, ctx.scale)
+        else:
+            if ctx.needs_input_grad[0] and ctx.image_coils == 1:
+                grad_image = fft_grid_adj(grad, smaps, scaling_coef, ctx.im_size, ctx.scale)
+            if ctx.needs_input_grad[1] or (ctx.needs_input_grad[0] and ctx.image_coils != 1):
+                per_coil = fft_grid_adj(grad, None, scaling_coef, ctx.im_size, ctx.scale)
+            if ctx.needs_input_grad[0] and ctx.image_coils != 1:
+                grad_image = per_coil * smaps.conj()
+            if ctx.needs_input_grad[1]:
+                grad_smaps = _sum_to(per_coil * image.conj(), ctx.smaps_shape)
+        return grad_image, grad_smaps, None, None, None, None
+
+
+class GridAdjoint(Function):
+    """grid (B, C, *K) [, smaps] -> sum_c crop(IFFT(grid)) * conj(scaling) * conj(smaps) * scale."""
+
+    @staticmethod
+    def forward(ctx, grid, smaps, scaling_coef, im_size, scale):
+        from .._nufft.fft import fft_grid_adj
+
+        ctx.im_size = tuple(im_size)
+        ctx.grid_size = tuple(grid.shape[2:])
+        ctx.scale = scale
+        ctx.smaps_shape = None if smaps is None else tuple(smaps.shape)
+        ctx.save_for_backward(grid if (smaps is not None and smaps.requires_grad) else None, smaps, scaling_coef)
+        return fft_grid_adj(grid, smaps, scaling_coef, im_size, scale)
+
+    @staticmethod
+    def backward(ctx, grad):
+        from .._nufft.fft import fft_grid_adj, fft_grid_fwd
+
+        grid, smaps, scaling_coef = ctx.saved_tensors
+        grad_grid = grad_smaps = None
+        if ctx.needs_input_grad[0]:
+            grad_grid = fft_grid_fwd(grad, smaps, scaling_coef, ctx.im_size, ctx.grid_size, ctx.scale)
+        if smaps is not None and ctx.needs_input_grad[1]:
+            per_coil = fft_grid_adj(grid, None, scaling_coef, ctx.im_size, ctx.scale)
+            grad_smaps = _sum_to(per_coil * grad.conj(), ctx.smaps_shape)
+        return grad_grid, grad_smaps, None, None, None

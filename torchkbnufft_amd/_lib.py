@@ -55,7 +55,8 @@ class InterpDesc(ctypes.Structure):
 _SIGNATURES = {
     "tkbn_abi_version": (ctypes.c_int, []),
     "tkbn_last_error": (ctypes.c_char_p, []),
-    "tkbn_interp_fwd": (ctypes.c_int, [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_vp]),
+    "tkbn_interp_fwd_workspace_bytes": (_c_i64, [ctypes.POINTER(InterpDesc)]),
+    "tkbn_interp_fwd": (ctypes.c_int, [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),
     "tkbn_interp_adj_workspace_bytes": (_c_i64, [ctypes.POINTER(InterpDesc), _c_i32]),
     "tkbn_interp_adj": (
         ctypes.c_int,
@@ -67,6 +68,18 @@ _SIGNATURES = {
         [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64, _c_i64,
          _c_vp, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp],
     ),
+    "tkbn_fft_supported": (ctypes.c_int, [_c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64)]),
+    "tkbn_fft_workspace_bytes": (_c_i64, [_c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64]),
+    "tkbn_fft_grid_fwd": (
+        ctypes.c_int,
+        [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64, _c_i64,
+         _c_vp, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp, _c_i64, _c_vp],
+    ),
+    "tkbn_fft_grid_adj": (
+        ctypes.c_int,
+        [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64,
+         _c_vp, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp, _c_i64, _c_vp],
+    ),
     "tkbn_crop_deapod": (
         ctypes.c_int,
         [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64,
@@ -74,7 +87,7 @@ _SIGNATURES = {
     ),
 }
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _lock = threading.Lock()
 _lib: Optional[ctypes.CDLL] = None

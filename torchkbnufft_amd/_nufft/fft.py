@@ -109,21 +109,118 @@ def crop_deapod(grid: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_
 
 def fft_and_scale(image: Tensor, scaling_coef: Tensor, im_size: Tensor, grid_size: Tensor, norm: Optional[str] = None) -> Tensor:
     """Oversampled FFT of ``image * scaling_coef`` (``fft.py:36-76``)."""
-    from .._autograd.fft import ApodPad
+    from .._autograd.fft import ApodPad, GridForward
 
     gs = host_values(grid_size)
     im = host_values(im_size)
     scale = _norm_scale(norm, gs)
+    if fused_supported(image.dtype, im, gs):
+        return GridForward.apply(image, None, scaling_coef, im, gs, scale)
     grid = ApodPad.apply(image, None, scaling_coef, im, gs, scale)
     return torch.fft.fftn(grid, dim=tuple(range(-len(gs), 0)), norm="backward")
 
 
 def ifft_and_scale(image: Tensor, scaling_coef: Tensor, im_size: Tensor, grid_size: Tensor, norm: Optional[str] = None) -> Tensor:
     """Unnormalised (or ortho) inverse FFT, crop, conjugate scaling (``fft.py:80-118``)."""
-    from .._autograd.fft import CropDeapod
+    from .._autograd.fft import CropDeapod, GridAdjoint
 
     gs = host_values(grid_size)
     im = host_values(im_size)
     scale = _norm_scale(norm, gs)
+    if fused_supported(image.dtype, im, gs):
+        return GridAdjoint.apply(image, None, scaling_coef, im, scale)
     x = torch.fft.ifftn(image, dim=tuple(range(-len(gs), 0)), norm="forward")
     return CropDeapod.apply(x, None, scaling_coef, im, scale)
+
+
+# ---------------------------------------------------------------------------
+# fused FFT path (complex64, grid sizes with prime factors 2, 3, 5)
+# ---------------------------------------------------------------------------
+_FUSED_OK = {}
+_FUSED_DISABLED = False
+
+
+def set_fused_fft(enabled: bool) -> None:
+    """Use the fused HIP FFT where the geometry allows (default) or always torch.fft."""
+    global _FUSED_DISABLED
+    _FUSED_DISABLED = not enabled
+
+
+def fused_supported(dtype: torch.dtype, im_size, grid_size) -> bool:
+    if _FUSED_DISABLED or dtype != torch.complex64:
+        return False
+    key = (tuple(int(v) for v in im_size), tuple(int(v) for v in grid_size))
+    ok = _FUSED_OK.get(key)
+    if ok is None:
+        d = len(key[0])
+        ok = _lib.load().tkbn_fft_supported(d, _lib.i64_array(key[0]), _lib.i64_array(key[1])) == _lib.TKBN_OK
+        _FUSED_OK[key] = ok
+    return ok
+
+
+def _fft_workspace(dev, d, im, gs, B, C):
+    n = _lib.load().tkbn_fft_workspace_bytes(d, _lib.i64_array(im), _lib.i64_array(gs), B, C)
+    if n < 0:
+        _lib.check(_lib.TKBN_ERR_INVALID, "tkbn_fft_workspace_bytes")
+    return torch.empty(max(int(n), 1), dtype=torch.uint8, device=dev), int(n)
+
+
+def fft_grid_fwd(image: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_size, grid_size, scale: float) -> Tensor:
+    """FFT(pad_end(image * smaps * scaling_coef)) * scale in one fused HIP pipeline."""
+    require_gpu(image, smaps, scaling_coef)
+    image = image.contiguous()
+    im = tuple(int(v) for v in im_size)
+    gs = tuple(int(v) for v in grid_size)
+    d = len(im)
+    B = image.shape[0]
+    C = smaps.shape[1] if smaps is not None else image.shape[1]
+    if tuple(image.shape[2:]) != im:
+        raise ValueError(f"image spatial shape {tuple(image.shape[2:])} != im_size {im}")
+    if smaps is not None:
+        smaps = smaps.to(image.dtype)
+        if smaps.shape[0] != B:
+            smaps = smaps.expand(B, *smaps.shape[1:])
+        smaps = smaps.contiguous()
+        if tuple(smaps.shape) != (B, C) + im:
+            raise ValueError(f"smaps shape {tuple(smaps.shape)} does not match image {tuple(image.shape)}")
+        if image.shape[1] not in (1, C):
+            raise ValueError("image coil dimension must be 1 or match smaps")
+    sc = scaling_coef.to(device=image.device, dtype=image.dtype).contiguous()
+    grid = torch.empty((B, C) + gs, dtype=image.dtype, device=image.device)
+    ws, nws = _fft_workspace(image.device, d, im, gs, B, C)
+    with torch.cuda.device(image.device):
+        st = _lib.load().tkbn_fft_grid_fwd(
+            _lib.dtype_code(image), d, _lib.i64_array(im), _lib.i64_array(gs), B, C, image.shape[1],
+            image.data_ptr(), smaps.data_ptr() if smaps is not None else None, sc.data_ptr(), float(scale),
+            grid.data_ptr(), ws.data_ptr(), nws, _lib.stream_of(image),
+        )
+    _lib.check(st, "tkbn_fft_grid_fwd")
+    return grid
+
+
+def fft_grid_adj(grid: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_size, scale: float) -> Tensor:
+    """sum_c crop(IFFT(grid)) * conj(scaling_coef) * conj(smaps_c) * scale (per coil without smaps)."""
+    require_gpu(grid, smaps, scaling_coef)
+    grid = grid.contiguous()
+    im = tuple(int(v) for v in im_size)
+    gs = tuple(grid.shape[2:])
+    d = len(im)
+    B, C = grid.shape[0], grid.shape[1]
+    if smaps is not None:
+        smaps = smaps.to(grid.dtype)
+        if smaps.shape[0] != B:
+            smaps = smaps.expand(B, *smaps.shape[1:])
+        smaps = smaps.contiguous()
+        if tuple(smaps.shape) != (B, C) + im:
+            raise ValueError(f"smaps shape {tuple(smaps.shape)} does not match data ({B}, {C}) + {im}")
+    sc = scaling_coef.to(device=grid.device, dtype=grid.dtype).contiguous()
+    out = torch.empty((B, 1 if smaps is not None else C) + im, dtype=grid.dtype, device=grid.device)
+    ws, nws = _fft_workspace(grid.device, d, im, gs, B, C)
+    with torch.cuda.device(grid.device):
+        st = _lib.load().tkbn_fft_grid_adj(
+            _lib.dtype_code(grid), d, _lib.i64_array(im), _lib.i64_array(gs), B, C, grid.data_ptr(),
+            smaps.data_ptr() if smaps is not None else None, sc.data_ptr(), float(scale), out.data_ptr(),
+            ws.data_ptr(), nws, _lib.stream_of(grid),
+        )
+    _lib.check(st, "tkbn_fft_grid_adj")
+    return out

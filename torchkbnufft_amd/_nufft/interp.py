@@ -18,6 +18,14 @@ from ._host import host_values, real_dtype_of, require_gpu
 
 # adjoint mode used when the caller does not ask for one
 _DEFAULT_ADJ_MODE = _lib.TKBN_ADJ_ATOMIC
+# samples x rows from which the forward uses the binned, LDS-staged gather
+_BINNED_FWD_MIN_WORK = 1 << 16
+
+
+def set_binned_forward_threshold(work: int) -> None:
+    """Samples x rows from which ``table_interp`` bins samples (0: always, huge: never)."""
+    global _BINNED_FWD_MIN_WORK
+    _BINNED_FWD_MIN_WORK = int(work)
 
 
 def set_deterministic(flag: bool) -> None:
@@ -109,9 +117,17 @@ def table_interp(
     B, C = image.shape[0], image.shape[1]
     desc = make_desc(image, omega, tables, n_shift, numpoints, table_oversamp, grid_size, B, C)
     kdata = torch.empty(B, C, omega.shape[-1], dtype=image.dtype, device=image.device)
+    lib = _lib.load()
+    ws, nws = None, 0
+    # binned LDS-staged gather once there is enough work to amortise the plan launches
+    if omega.shape[-1] * B * C >= _BINNED_FWD_MIN_WORK:
+        nws = int(lib.tkbn_interp_fwd_workspace_bytes(ctypes.byref(desc)))
+        if nws > 0:
+            ws = torch.empty(nws, dtype=torch.uint8, device=image.device)
     with torch.cuda.device(image.device):
-        st = _lib.load().tkbn_interp_fwd(
-            ctypes.byref(desc), image.data_ptr(), omega.data_ptr(), kdata.data_ptr(), _lib.stream_of(image)
+        st = lib.tkbn_interp_fwd(
+            ctypes.byref(desc), image.data_ptr(), omega.data_ptr(), kdata.data_ptr(),
+            ws.data_ptr() if ws is not None else None, nws if ws is not None else 0, _lib.stream_of(image),
         )
     _lib.check(st, "tkbn_interp_fwd")
     return kdata

@@ -252,6 +252,11 @@ int interp_domega_t(const tkbn_interp_desc* d, const void* grid, const void* gou
   TKBN_DISPATCH_DJ(d->ndim, max_numpoints(d), domega_launch, g, grid, gout, omega, out, s);
 }
 
+// binned forward (tkbn_spread.hip)
+int64_t interp_fwd_workspace_bytes(const tkbn_interp_desc* d);
+int interp_fwd_tiled(const tkbn_interp_desc* d, const void* grid, const void* omega, void* kdata, void* ws,
+                     hipStream_t s);
+
 }  // namespace tkbn
 
 using namespace tkbn;
@@ -262,13 +267,28 @@ int tkbn_abi_version(void) { return TKBN_ABI_VERSION; }
 
 const char* tkbn_last_error(void) { return g_err; }
 
+int64_t tkbn_interp_fwd_workspace_bytes(const tkbn_interp_desc* desc) {
+  if (validate_desc(desc) != TKBN_OK) return -1;
+  return interp_fwd_workspace_bytes(desc);
+}
+
 int tkbn_interp_fwd(const tkbn_interp_desc* desc, const void* grid, const void* omega, void* kdata,
-                    void* stream) {
+                    void* workspace, int64_t workspace_bytes, void* stream) {
   int st = validate_desc(desc);
   if (st != TKBN_OK) return st;
   if (desc->batch == 0 || desc->coils == 0 || desc->klength == 0) return TKBN_OK;
   TKBN_REQUIRE(grid && omega && kdata, TKBN_ERR_INVALID, "NULL data pointer");
+  TKBN_REQUIRE(desc->klength < (1LL << 31), TKBN_ERR_INVALID, "klength must be < 2^31");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (workspace != nullptr && workspace_bytes > 0) {
+    const int64_t need = interp_fwd_workspace_bytes(desc);
+    TKBN_REQUIRE(need > 0, TKBN_ERR_UNSUPPORTED, "binned forward not available for this geometry");
+    TKBN_REQUIRE(workspace_bytes >= need, TKBN_ERR_WORKSPACE, "workspace of %lld bytes needed, %lld given",
+                 (long long)need, (long long)workspace_bytes);
+    TKBN_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 15) == 0, TKBN_ERR_INVALID,
+                 "workspace must be 16-byte aligned");
+    return interp_fwd_tiled(desc, grid, omega, kdata, workspace, s);
+  }
   if (desc->dtype == TKBN_F32) return interp_fwd_t<float>(desc, grid, omega, kdata, s);
   return interp_fwd_t<double>(desc, grid, omega, kdata, s);
 }

@@ -6,9 +6,9 @@ from typing import List, Optional
 import torch
 from torch import Tensor
 
-from .._autograd.fft import ApodPad, CropDeapod
+from .._autograd.fft import ApodPad, CropDeapod, GridAdjoint, GridForward
 from .._nufft._host import host_values
-from .._nufft.fft import _norm_scale
+from .._nufft.fft import _norm_scale, fused_supported
 from .interp import kb_table_interp, kb_table_interp_adjoint
 
 
@@ -16,8 +16,11 @@ def _nufft_forward(image, smaps, scaling_coef, im_size, grid_size, omega, tables
                    table_oversamp, offsets, norm):
     gs = host_values(grid_size)
     im = host_values(im_size)
-    grid = ApodPad.apply(image, smaps, scaling_coef, im, gs, _norm_scale(norm, gs))
-    grid = torch.fft.fftn(grid, dim=tuple(range(-len(gs), 0)), norm="backward")
+    if fused_supported(image.dtype, im, gs):
+        grid = GridForward.apply(image, smaps, scaling_coef, im, gs, _norm_scale(norm, gs))
+    else:
+        grid = ApodPad.apply(image, smaps, scaling_coef, im, gs, _norm_scale(norm, gs))
+        grid = torch.fft.fftn(grid, dim=tuple(range(-len(gs), 0)), norm="backward")
     return kb_table_interp(grid, omega, tables, n_shift, numpoints, table_oversamp, offsets)
 
 
@@ -26,6 +29,8 @@ def _nufft_adjoint(data, smaps, scaling_coef, im_size, grid_size, omega, tables,
     gs = host_values(grid_size)
     im = host_values(im_size)
     grid = kb_table_interp_adjoint(data, omega, tables, n_shift, numpoints, table_oversamp, offsets, gs)
+    if fused_supported(grid.dtype, im, gs):
+        return GridAdjoint.apply(grid, smaps, scaling_coef, im, _norm_scale(norm, gs))
     grid = torch.fft.ifftn(grid, dim=tuple(range(-len(gs), 0)), norm="forward")
     return CropDeapod.apply(grid, smaps, scaling_coef, im, _norm_scale(norm, gs))
 
