@@ -139,7 +139,8 @@ __device__ void build_twiddles(float2* tw, int K) {
 // The pass is in place: every thread reads and transforms all its butterflies
 // (at most kMaxBfly, guaranteed by the host's choice of ntr), the group syncs, then
 // every thread writes them back.
-constexpr int kMaxBfly = 8;
+constexpr int kMaxBfly = 2;
+constexpr int kFftThreads = 1024;  // 16 waves per group: the stash stays small (<=128 VGPRs)
 
 template <bool INV, int R, bool T_MAJOR>
 __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int K, int ntr, int st, int sk, int Ns) {
@@ -232,7 +233,7 @@ struct Extents {
 // the group loops over coils.
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     fft_last_kernel(const FftGeom g, const float2* __restrict__ src, const float2* __restrict__ smaps,
                     const float2* __restrict__ scaling, float2* __restrict__ dst, int lines_per_group) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -279,9 +280,9 @@ __global__ void __launch_bounds__(256)
     const int64_t line = blockIdx.x;
     const int64_t b = line / Q, q = line - b * Q;
     const int cc = lines_per_group;
-    // outputs owned by this thread: n = threadIdx.x + i * blockDim.x
-    float2 acc[4];
-    for (int i = 0; i < 4; ++i) acc[i] = make_float2(0.f, 0.f);
+    // output owned by this thread: n = threadIdx.x (the host guarantees N <= blockDim)
+    const int n = threadIdx.x;
+    float2 acc = make_float2(0.f, 0.f);
     for (int64_t c0 = 0; c0 < g.C; c0 += cc) {
       const int nc = (int)((c0 + cc <= g.C) ? cc : g.C - c0);
       for (int idx = threadIdx.x; idx < nc * K; idx += blockDim.x) {
@@ -291,37 +292,29 @@ __global__ void __launch_bounds__(256)
       __syncthreads();
       lds_fft<true, true>(buf, tw, g.plan[D - 1], nc, K, 1);
       if (smaps) {
-        for (int i = 0; i < 4; ++i) {
-          const int n = threadIdx.x + i * blockDim.x;
-          if (n >= N) break;
+        if (n < N) {
           const float2 s = scaling[q * N + n];
           for (int l = 0; l < nc; ++l) {
             float2 v = cmul_conj(buf[l * K + n], s);
             v.x *= g.scale;
             v.y *= g.scale;
             const float2 w = cmul_conj(v, smaps[((b * g.C + c0 + l) * Q + q) * N + n]);
-            acc[i].x += w.x;
-            acc[i].y += w.y;
+            acc.x += w.x;
+            acc.y += w.y;
           }
         }
       } else {
         for (int idx = threadIdx.x; idx < nc * N; idx += blockDim.x) {
-          const int l = idx / N, n = idx - l * N;
-          float2 v = cmul_conj(buf[l * K + n], scaling[q * N + n]);
+          const int l = idx / N, nn = idx - l * N;
+          float2 v = cmul_conj(buf[l * K + nn], scaling[q * N + nn]);
           v.x *= g.scale;
           v.y *= g.scale;
-          dst[((b * g.C + c0 + l) * Q + q) * N + n] = v;
+          dst[((b * g.C + c0 + l) * Q + q) * N + nn] = v;
         }
       }
       __syncthreads();
     }
-    if (smaps) {
-      for (int i = 0; i < 4; ++i) {
-        const int n = threadIdx.x + i * blockDim.x;
-        if (n >= N) break;
-        dst[(b * Q + q) * N + n] = acc[i];
-      }
-    }
+    if (smaps && n < N) dst[(b * Q + q) * N + n] = acc;
   }
 }
 
@@ -332,7 +325,7 @@ __global__ void __launch_bounds__(256)
 // Forward: n_in = N[d] zero-padded to K; adjoint (INV): n_out = N[d] (crop).
 // ---------------------------------------------------------------------------
 template <bool INV>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     fft_mid_kernel(const FftPlan1D pl, int64_t rows, int n_in, int n_out, int64_t inner, int ncol,
                    const float2* __restrict__ src, float2* __restrict__ dst) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -401,7 +394,7 @@ static int min_radix(const FftPlan1D& pl) {
 
 // transforms per group so that no thread has more than kMaxBfly butterflies
 static int max_transforms(const FftPlan1D& pl) {
-  return (int)((int64_t)kMaxBfly * 256 * min_radix(pl) / pl.K);
+  return (int)((int64_t)kMaxBfly * kFftThreads * min_radix(pl) / pl.K);
 }
 
 constexpr int kFftLdsBudget = 60 * 1024;
@@ -426,7 +419,7 @@ static int check_fft(int32_t ndim, const int64_t* im, const int64_t* gs, FftGeom
   }
   // the last-dimension pass holds at least one line plus twiddles
   TKBN_REQUIRE((2LL * g->K[ndim - 1]) * 8 <= kFftLdsBudget, TKBN_ERR_UNSUPPORTED, "last grid dim too long");
-  TKBN_REQUIRE(g->N[ndim - 1] <= 4 * 256, TKBN_ERR_UNSUPPORTED, "last image dim must be <= 1024");
+  TKBN_REQUIRE(g->N[ndim - 1] <= kFftThreads, TKBN_ERR_UNSUPPORTED, "last image dim must be <= %d", kFftThreads);
   return TKBN_OK;
 }
 
@@ -444,9 +437,9 @@ static int launch_mid(bool inv, const FftPlan1D& pl, int64_t rows, int n_in, int
   TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_INVALID, "too many FFT columns");
   const size_t lds = (size_t)(pl.K + (int64_t)pl.K * ncol) * sizeof(float2);
   if (inv)
-    fft_mid_kernel<true><<<dim3((unsigned)groups), dim3(256), lds, s>>>(pl, rows, n_in, n_out, inner, ncol, src, dst);
+    fft_mid_kernel<true><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(pl, rows, n_in, n_out, inner, ncol, src, dst);
   else
-    fft_mid_kernel<false><<<dim3((unsigned)groups), dim3(256), lds, s>>>(pl, rows, n_in, n_out, inner, ncol, src, dst);
+    fft_mid_kernel<false><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(pl, rows, n_in, n_out, inner, ncol, src, dst);
   TKBN_LAUNCH_CHECK("fft_mid_kernel");
   return TKBN_OK;
 }
@@ -532,7 +525,7 @@ int tkbn_fft_grid_fwd(int32_t dtype, int32_t ndim, const int64_t* im_size, const
     const int64_t groups = ceil_div(nlines, lpg);
     float2* out = D == 1 ? static_cast<float2*>(grid) : bufa;
     const size_t lds = (size_t)(K + (int64_t)lpg * K) * sizeof(float2);
-    fft_last_kernel<0><<<dim3((unsigned)groups), dim3(256), lds, s>>>(
+    fft_last_kernel<0><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(
         g, static_cast<const float2*>(image), static_cast<const float2*>(smaps),
         static_cast<const float2*>(scaling_coef), out, lpg);
     TKBN_LAUNCH_CHECK("fft_last_kernel<fwd>");
@@ -605,7 +598,7 @@ int tkbn_fft_grid_adj(int32_t dtype, int32_t ndim, const int64_t* im_size, const
     const size_t lds = (size_t)(K + (int64_t)cc * K) * sizeof(float2);
     if (smaps) {
       const int64_t groups = g.B * Q;
-      fft_last_kernel<1><<<dim3((unsigned)groups), dim3(256), lds, s>>>(
+      fft_last_kernel<1><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(
           g, cur, static_cast<const float2*>(smaps), static_cast<const float2*>(scaling_coef),
           static_cast<float2*>(image), cc);
     } else {
@@ -614,7 +607,7 @@ int tkbn_fft_grid_adj(int32_t dtype, int32_t ndim, const int64_t* im_size, const
       g1.B = g.B * g.C;
       g1.C = 1;
       const int64_t groups = g1.B * Q;
-      fft_last_kernel<1><<<dim3((unsigned)groups), dim3(256), lds, s>>>(
+      fft_last_kernel<1><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(
           g1, cur, nullptr, static_cast<const float2*>(scaling_coef), static_cast<float2*>(image), 1);
     }
     TKBN_LAUNCH_CHECK("fft_last_kernel<adj>");

@@ -49,6 +49,8 @@ struct TileGeom {
   // workspace offsets (in int32 units) of one trajectory's plan
   int64_t o_key, o_perm, o_count, o_start, o_cursor, o_wstart, o_work, o_nwork, plan_ints;
   int64_t o_maxabs_bytes;  // byte offset of the max|kdata| slot (8 bytes)
+  int64_t nkeys;           // multi-coil plan: (tile, footprint column) keys
+  int64_t o_rec_bytes;     // multi-coil plan: byte offset of the entry records
 };
 
 // dynamic LDS per group; with the ~350 bytes of static LDS the total stays under 64 KiB
@@ -88,25 +90,50 @@ __device__ __forceinline__ int dim_tiles(const TileGeom& tg, int d, int bm, int*
   return n;
 }
 
-// atomicAdd(ptr + key, 1) for every lane with `valid`, one atomic per distinct key in
-// the wave; returns the value this lane's increment would have seen (lanes with equal
-// keys are ranked in lane order).  Every active lane of the wave must call it.
-__device__ __forceinline__ int wave_aggregated_inc(int* ptr, int key, bool valid) {
+// Groups the wave's valid lanes by key (ballots and shuffles only): the lowest lane of
+// each group is its leader; `rank` is the lane's position inside its group and `cnt`
+// (leaders only) the group size.
+__device__ __forceinline__ void wave_group_by_key(int key, bool valid, int& leader_of, int& rank, int& cnt) {
   const int lane = threadIdx.x & 63;
   unsigned long long pending = __ballot(valid);
-  int res = -1;
+  leader_of = lane;
+  rank = 0;
+  cnt = 0;
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
     const int lkey = __shfl(key, leader, 64);
+    // keys of earlier groups differ from lkey, so `mine` only selects pending lanes
     const bool mine = valid && key == lkey;
     const unsigned long long same = __ballot(mine) & pending;
-    int old = 0;
-    if (lane == leader) old = atomicAdd(ptr + lkey, __popcll(same));
-    old = __shfl(old, leader, 64);
-    if (mine) res = old + __popcll(same & ((1ull << lane) - 1ull));
+    if (lane == leader) cnt = __popcll(same);
+    if (mine) {
+      leader_of = leader;
+      rank = __popcll(same & ((1ull << lane) - 1ull));
+    }
     pending &= ~same;
   }
-  return res;
+}
+
+// atomicAdd(ptr + key, 1) for every lane with `valid`: one atomic per distinct key in
+// the wave, all issued by one instruction.  Returns the value this lane's increment
+// would have seen (lanes with equal keys are ranked in lane order), -1 if not valid.
+// Every active lane of the wave must call it.
+__device__ __forceinline__ int wave_aggregated_inc(int* ptr, int key, bool valid) {
+  const int lane = threadIdx.x & 63;
+  int leader_of, rank, cnt;
+  wave_group_by_key(key, valid, leader_of, rank, cnt);
+  int old = 0;
+  if (valid && leader_of == lane) old = atomicAdd(ptr + key, cnt);
+  old = __shfl(old, leader_of, 64);
+  return valid ? old + rank : -1;
+}
+
+// the same increments when the old values are not needed (no return trip)
+__device__ __forceinline__ void wave_aggregated_add(int* ptr, int key, bool valid) {
+  const int lane = threadIdx.x & 63;
+  int leader_of, rank, cnt;
+  wave_group_by_key(key, valid, leader_of, rank, cnt);
+  if (valid && leader_of == lane) atomicAdd(ptr + key, cnt);
 }
 
 template <typename T, int D>
@@ -153,7 +180,7 @@ __global__ void __launch_bounds__(256)
   int* count = ws + b * tg.plan_ints + tg.o_count;
   int tl[3][3], ntl[3];
   sample_tiles<T, D>(g, tg, om, m, live, tl, ntl);
-  for_each_tile<D>(tg, tl, ntl, live, [&](int key, bool valid) { wave_aggregated_inc(count, key, valid); });
+  for_each_tile<D>(tg, tl, ntl, live, [&](int key, bool valid) { wave_aggregated_add(count, key, valid); });
 }
 
 template <typename T, int D>
@@ -279,19 +306,17 @@ __global__ void __launch_bounds__(256)
   const int64_t r1 = r0 + tg.CC < nrows ? r0 + tg.CC : nrows;
   int origin[3] = {0, 0, 0};
   for (int d = 0; d < D; ++d) origin[d] = ((t / tg.bstride[d]) % tg.nb[d]) * tg.T[d];
-  for (int64_t r = r0; r < r1; ++r) {
-    cplx<T>* gp = grid + (rowbase + r) * g.G;
-    for (int cell = threadIdx.x; cell < tg.tcells; cell += blockDim.x) {
-      int64_t gi = 0;
-      bool in = true;
-      for (int d = 0; d < D; ++d) {
-        const int l = (cell / tg.lstride[d]) % tg.T[d];
-        const int k = origin[d] + l;
-        in = in && k < g.K[d];
-        gi += (int64_t)k * g.stride[d];
-      }
-      if (in) gp[gi] = make_c<T>(0, 0);
+  for (int cell = threadIdx.x; cell < tg.tcells; cell += blockDim.x) {
+    int64_t gi = 0;
+    bool in = true;
+    for (int d = 0; d < D; ++d) {
+      const int l = (cell / tg.lstride[d]) % tg.T[d];
+      const int k = origin[d] + l;
+      in = in && k < g.K[d];
+      gi += (int64_t)k * g.stride[d];
     }
+    if (!in) continue;
+    for (int64_t r = r0; r < r1; ++r) grid[(rowbase + r) * g.G + gi] = make_c<T>(0, 0);
   }
 }
 
@@ -464,45 +489,231 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// multi-coil spread with lanes = coil rows (2-D, complex64, J <= 6)
-//   The sample is wave-uniform, so all 64 lanes add the same tap to 64 distinct
-//   addresses (one per row): no intra-wave conflicts and no divergence.  Tap weights
-//   and in-tile indices are computed lane-parallel for 32 candidates at a time and
-//   staged in LDS.  Accumulators are planar (re, im) with a row pitch of 65 so the
-//   cell-major flush reads LDS without bank conflicts and writes the grid coalesced.
+// multi-coil spread with register accumulators (2-D, complex64, J <= 6)
+//   One group (8 waves) per (16 x 16 tile, split, chunk of 64 coil rows); lanes are
+//   coil rows.  Wave w owns tile rows 2w and 2w+1: their 2 x 16 cells live in its
+//   registers, so every cell has exactly one writer and no atomics are needed.
+//   The plan stores, for every (sample, tile) entry, a record with the sample's tap
+//   weights (phase folded into dimension 0, both conjugated) and the footprint's first
+//   row r0 and column c0 relative to the tile, and orders each tile's entries by c0.
+//   The kernel stages 64 entries at a time (records, and the entries' data with
+//   lanes = entries so consecutive trajectory samples load coalesced), then walks the
+//   c0 groups in order: inside a group the column offsets are compile-time constants,
+//   so every accumulator index is fixed and the taps are plain FMAs.
+//   Footprints are contiguous inside a tile when every grid extent is at least
+//   tile + J - 1 cells (a wrapped footprint then never meets one tile twice);
+//   smaller grids use the sample-lane kernel.
 // ---------------------------------------------------------------------------
-constexpr int kCoilPitch = 65;
-constexpr int kCoilBatch = 32;
+constexpr int kRegRowsPerWave = 2;
+constexpr int kRegWaves = 8;
+constexpr int kRegThreads = 64 * kRegWaves;
+constexpr int kRegT0 = kRegRowsPerWave * kRegWaves;  // 16
+constexpr int kRegT1 = 16;
+constexpr int kRegBatch = 64;  // staged entries
+constexpr int kRegPitch = 65;  // float2 per staged entry: 64 rows + 1 (bank spread)
+constexpr int kRegMinC0 = -5;  // first footprint column relative to the tile, J <= 6
+constexpr int kRegKeys = kRegT1 - kRegMinC0;  // c0 values per tile: -5 .. 15
 
-struct CoilStage {
-  float2 w0[6];
-  float2 w1[6];
-  float2 phase;
-  int m;
-  signed char l0[6];
-  signed char l1[6];
+struct CoilRec {
+  float2 w0[6];  // conj(phase * w0[j]) along dimension 0
+  float2 w1[6];  // conj(w1[j]) along dimension 1
+  int r0;        // first footprint row relative to the tile
+  int c0;        // first footprint column relative to the tile
   int pad[2];
 };
-static_assert(sizeof(CoilStage) == 128, "stage record is 128 bytes");
+static_assert(sizeof(CoilRec) == 112, "plan record is 112 bytes");
 
-// accumulator plane length, a multiple of 4 elements so the following plane and the
-// stage records stay 16-byte aligned
-__host__ __device__ inline int coil_plane_elems(int tcells) { return (tcells * kCoilPitch + 3) & ~3; }
+// Tiles met along one dimension by the footprint bm .. bm+J-1 (mod K), in footprint
+// order, with the footprint's first cell relative to each tile's origin.  The runs of
+// one tile are contiguous because K >= T + J - 1.
+__device__ __forceinline__ int coil_dim_tiles(int K, int J, int T, int bm, int* tl, int* off) {
+  int n = 0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (j < J) {
+      int c = bm + j;
+      if (c >= K) c -= K;  // bm < K and J < K
+      const int tile = c / T;
+      if (n == 0 || tl[n - 1] != tile) {
+        tl[n] = tile;
+        off[n] = c - tile * T - j;
+        ++n;
+      }
+    }
+  }
+  return n;
+}
 
-template <bool DET>
+// Walks a sample's (tile0, tile1) pairs in a wave-uniform loop; fn(key, pair0, pair1, valid).
+template <typename F>
+__device__ __forceinline__ void coil_for_each_entry(const TileGeom& tg, const int (&tl)[2][3], const int (&off)[2][3],
+                                                    const int (&n)[2], bool live, F fn) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const bool valid = live && a < n[0] && c < n[1];
+      const int key = valid ? ((tl[0][a] * tg.nb[1] + tl[1][c]) * kRegKeys + off[1][c] - kRegMinC0) : 0;
+      if (__ballot(valid)) fn(key, a, c, valid);
+    }
+}
+
+__device__ __forceinline__ void coil_sample_tiles(const Geom<float>& g, const TileGeom& tg, const float* om, int m,
+                                                  bool live, int (&tl)[2][3], int (&off)[2][3], int (&n)[2]) {
+  n[0] = n[1] = 0;
+#pragma unroll
+  for (int d = 0; d < 2; ++d)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) tl[d][q] = off[d][q] = 0;
+  if (!live) return;
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int bm = (int)sample_base_wrapped<float>(g, d, om[d * g.M + m]);
+    n[d] = coil_dim_tiles(g.K[d], g.J[d], tg.T[d], bm, tl[d], off[d]);
+  }
+}
+
 __global__ void __launch_bounds__(256)
-    spread_coils_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ kdata,
-                        const float* __restrict__ omega, const int* __restrict__ ws,
-                        const void* __restrict__ maxabs_slot, float2* __restrict__ grid) {
-  using A = typename AccOf<float, DET>::type;
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  const int tcells = tg.tcells;
-  const int plane = coil_plane_elems(tcells);
-  A* are = reinterpret_cast<A*>(smem_raw);
-  A* aim = are + plane;
-  CoilStage* stage_all = reinterpret_cast<CoilStage*>(aim + plane);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  CoilStage* stage = stage_all + wave * kCoilBatch;
+    coil_bin_count_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
+                          int* __restrict__ ws) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = m < g.M;
+  int tl[2][3], off[2][3], n[2];
+  coil_sample_tiles(g, tg, omega, m, live, tl, off, n);
+  int* count = ws + tg.o_count;
+  coil_for_each_entry(tg, tl, off, n, live,
+                      [&](int key, int, int, bool valid) { wave_aggregated_add(count, key, valid); });
+}
+
+__global__ void __launch_bounds__(256)
+    coil_bin_scatter_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
+                            int* __restrict__ ws) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = m < g.M;
+  int tl[2][3], off[2][3], n[2];
+  coil_sample_tiles(g, tg, omega, m, live, tl, off, n);
+  CoilRec rec;
+  if (live) {
+    float2 w[6];
+    int mdx[6];
+    const float theta = sample_theta<float, 2>(g, omega, m);
+    const float2 ph = make_float2(cosf(theta), sinf(theta));
+    sample_taps_md<float, 6>(g, 0, omega[m], w, mdx);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const float2 pw = cmul(ph, w[j]);
+      rec.w0[j] = make_float2(pw.x, -pw.y);
+    }
+    sample_taps_md<float, 6>(g, 1, omega[g.M + m], w, mdx);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) rec.w1[j] = make_float2(w[j].x, -w[j].y);
+  }
+  rec.pad[0] = rec.pad[1] = 0;
+  int* plan = ws;
+  CoilRec* recs = reinterpret_cast<CoilRec*>(reinterpret_cast<char*>(ws) + tg.o_rec_bytes);
+  coil_for_each_entry(tg, tl, off, n, live, [&](int key, int a, int c, bool valid) {
+    const int pos = wave_aggregated_inc(plan + tg.o_cursor, key, valid);
+    if (valid) {
+      plan[tg.o_perm + pos] = m;
+      rec.r0 = off[0][a];
+      rec.c0 = off[1][c];
+      recs[pos] = rec;
+    }
+  });
+}
+
+// Scan of the (tile, c0) counts: per-tile totals are scanned across the block, then
+// each tile's keys are offset locally.  The totals also give the splits and work list.
+__global__ void __launch_bounds__(1024) coil_plan_scan_kernel(const TileGeom tg, int* __restrict__ ws) {
+  __shared__ int sbuf[1024];
+  int* plan = ws;
+  const int* count = plan + tg.o_count;
+  int* start = plan + tg.o_start;
+  int* cursor = plan + tg.o_cursor;
+  int* wstart = plan + tg.o_wstart;
+  int* work = plan + tg.o_work;
+  const int NB = tg.NB;
+  // 1. tile totals (work[0..NB) as scratch), scanned into cursor[0..NB] (scratch)
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int* c = count + (int64_t)t * kRegKeys;
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < kRegKeys; ++k) tot += c[k];
+    work[t] = tot;
+  }
+  __syncthreads();
+  block_exclusive_scan(work, cursor, NB, sbuf);
+  // 2. key starts: tile start + prefix over the tile's keys
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int* c = count + (int64_t)t * kRegKeys;
+    int run = cursor[t];
+    int* st = start + (int64_t)t * kRegKeys;
+#pragma unroll
+    for (int k = 0; k < kRegKeys; ++k) {
+      st[k] = run;
+      run += c[k];
+    }
+  }
+  if (threadIdx.x == 0) start[tg.nkeys] = cursor[NB];
+  __syncthreads();
+  // 3. cursors, splits and the work list
+  for (int64_t i = threadIdx.x; i < tg.nkeys; i += blockDim.x) cursor[i] = start[i];
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int v = start[(int64_t)(t + 1) * kRegKeys] - start[(int64_t)t * kRegKeys];
+    work[t] = v <= tg.target ? 1 : (v + tg.target - 1) / tg.target;
+  }
+  __syncthreads();
+  block_exclusive_scan(work, wstart, NB, sbuf);
+  const int cap = (int)tg.maxwork;
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int a = wstart[t], e = min(wstart[t + 1], cap);
+    for (int w = a; w < e; ++w) work[w] = t;
+  }
+  if (threadIdx.x == 0) plan[tg.o_nwork] = min(wstart[NB], cap);
+}
+
+// acc[C0 + j] += u * w1[j] for the footprint columns that fall inside the tile
+template <int C0>
+__device__ __forceinline__ void reg_add_row(float2 (&acc)[kRegT1], const float2 u, const float2 (&w1)[6]) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (C0 + j >= 0 && C0 + j < kRegT1) {
+      acc[C0 + j].x = fmaf(u.x, w1[j].x, acc[C0 + j].x);
+      acc[C0 + j].x = fmaf(-u.y, w1[j].y, acc[C0 + j].x);
+      acc[C0 + j].y = fmaf(u.x, w1[j].y, acc[C0 + j].y);
+      acc[C0 + j].y = fmaf(u.y, w1[j].x, acc[C0 + j].y);
+    }
+  }
+}
+
+// one entry with footprint column C0 whose rows meet the wave's rows; the weights are
+// read from the plan record at a wave-uniform address (scalar loads)
+template <int C0>
+__device__ __forceinline__ void reg_entry(float2 (&acc)[kRegRowsPerWave][kRegT1], const CoilRec* __restrict__ rec,
+                                          const float2 v, int ja, int J0) {
+  float2 w1[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) w1[j] = rec->w1[j];
+#pragma unroll
+  for (int h = 0; h < kRegRowsPerWave; ++h) {
+    const int j0 = ja + h;
+    if (j0 >= 0 && j0 < J0) {
+      const float2 w0 = rec->w0[j0];
+      float2 u;
+      u.x = fmaf(v.x, w0.x, -v.y * w0.y);
+      u.y = fmaf(v.x, w0.y, v.y * w0.x);
+      reg_add_row<C0>(acc[h], u, w1);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_eu(4)))
+    spread_coil_regs_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ kdata,
+                            const int* __restrict__ ws, float2* __restrict__ grid) {
+  __shared__ __attribute__((aligned(16))) float2 vals[kRegBatch * kRegPitch];
+  // wave-uniform in a scalar register, so the row tests below are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
 
   const int* plan = ws;
   const int64_t wi = blockIdx.x;
@@ -512,110 +723,106 @@ __global__ void __launch_bounds__(256)
   const int split = (int)(wi - wstart[t]);
   const int nsplit = wstart[t + 1] - wstart[t];
   const int64_t nrows = g.B * g.C;
-  const int64_t r0 = (int64_t)blockIdx.y * 64;
-  const int ncc = (int)((r0 + 64 < nrows ? r0 + 64 : nrows) - r0);
-  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * tg.T[0];
-  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * tg.T[1];
-  const int V = plan[tg.o_count + t];
-  const int cstart = plan[tg.o_start + t];
+  const int64_t rbase = (int64_t)blockIdx.y * 64;
+  const int ncc = (int)((rbase + 64 < nrows ? rbase + 64 : nrows) - rbase);
+  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kRegT0;
+  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kRegT1;
+  const int cstart = plan[tg.o_start + t * kRegKeys];
+  const int V = plan[tg.o_start + (t + 1) * kRegKeys] - cstart;
   const int i0 = split * tg.target;
   const int i1 = min(i0 + tg.target, V);
+  const int* perm = plan + tg.o_perm + cstart;
+  const CoilRec* recs = reinterpret_cast<const CoilRec*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) + cstart;
+  const int J0 = g.J[0];
+  const int myrow = wave * kRegRowsPerWave;  // first owned row, relative to the tile
 
-  for (int i = threadIdx.x; i < 2 * plane; i += blockDim.x) are[i] = A(0);
-  float scale = 1.f;
-  if constexpr (DET) {
-    float amax;
-    memcpy(&amax, maxabs_slot, sizeof(float));
-    const double bound = 2.0 * (double)amax * (double)(V > 0 ? V : 1);
-    int e = 50;
-    if (bound > 0 && bound < 1e300) e = 60 - (ilogb(bound) + 1);
-    e = e > 100 ? 100 : e;
-    scale = ldexpf(1.f, e);
-  }
-  __syncthreads();
+  float2 acc[kRegRowsPerWave][kRegT1];
+#pragma unroll
+  for (int h = 0; h < kRegRowsPerWave; ++h)
+#pragma unroll
+    for (int c = 0; c < kRegT1; ++c) acc[h][c] = make_float2(0.f, 0.f);
 
-  const int* perm = plan + tg.o_perm;
-  const int niter = (i1 - i0 + 4 * kCoilBatch - 1) / (4 * kCoilBatch);
-  for (int it = 0; it < niter; ++it) {
-    const int base = i0 + (it * 4 + wave) * kCoilBatch;
-    const int nb = min(kCoilBatch, i1 - base);  // may be <= 0 for trailing waves
-    if (lane < nb) {
-      const int m = perm[cstart + base + lane];
-      CoilStage& st = stage[lane];
-      float2 w[6];
-      int mdx[6];
-      sample_taps_md<float, 6>(g, 0, omega[m], w, mdx);
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        st.w0[j] = w[j];
-        const int l = mdx[j] - o0;
-        st.l0[j] = (signed char)((mdx[j] >= 0 && l >= 0 && l < tg.T[0]) ? l : -1);
-      }
-      sample_taps_md<float, 6>(g, 1, omega[g.M + m], w, mdx);
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        st.w1[j] = w[j];
-        const int l = mdx[j] - o1;
-        st.l1[j] = (signed char)((mdx[j] >= 0 && l >= 0 && l < tg.T[1]) ? l : -1);
-      }
-      const float theta = sample_theta<float, 2>(g, omega, m);
-      st.phase = make_float2(cosf(theta), sinf(theta));
-      st.m = m;
+  // Software pipeline: while a batch is processed, the next batch's data and entry
+  // offsets are loaded into registers, and the trajectory indices one batch further.
+  int2 prc = make_int2(0, 0);  // (r0, c0) of entry `lane` of the next batch
+  float2 pv[8];
+  auto prefetch = [&](int b, int mb) {
+    const int n = min(kRegBatch, i1 - b);
+    prc = make_int2(1000, 1000);
+    if (lane < n) {
+      const CoilRec* r = recs + b + lane;
+      prc = make_int2(r->r0, r->c0);
     }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int row = wave * 8 + q;
+      pv[q] = make_float2(0.f, 0.f);
+      if (lane < n && row < ncc) pv[q] = kdata[(rbase + row) * g.M + mb];
+    }
+  };
+  int mnext = 0;
+  if (i0 < i1) {
+    const int m0 = lane < min(kRegBatch, i1 - i0) ? perm[i0 + lane] : 0;
+    prefetch(i0, m0);
+    const int b1 = i0 + kRegBatch;
+    if (b1 < i1 && lane < min(kRegBatch, i1 - b1)) mnext = perm[b1 + lane];
+  }
+  for (int base = i0; base < i1; base += kRegBatch) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) vals[lane * kRegPitch + wave * 8 + q] = pv[q];
+    const int r0l = prc.x, c0l = prc.y;  // entry `lane` of this batch (1000: none)
     __syncthreads();
-    for (int sidx = 0; sidx < nb; ++sidx) {
-      const CoilStage& st = stage[sidx];
-      float2 v = make_float2(0.f, 0.f);
-      if (lane < ncc) v = cmul_conj(kdata[(r0 + lane) * g.M + st.m], st.phase);
+    const int b1 = base + kRegBatch;
+    if (b1 < i1) {
+      prefetch(b1, mnext);
+      const int b2 = b1 + kRegBatch;
+      mnext = (b2 < i1 && lane < min(kRegBatch, i1 - b2)) ? perm[b2 + lane] : 0;
+    }
+    // entries whose footprint rows meet this wave's rows
+    const int jal = myrow - r0l;
+    const unsigned long long hits = __ballot(jal + kRegRowsPerWave > 0 && jal < J0);
+    const CoilRec* brec = recs + base;
+    // entries are ordered by c0: walk each c0 value's hits with fixed column offsets
+#define TKBN_REG_GROUP(C0)                                                       \
+  {                                                                              \
+    unsigned long long mk = __ballot(c0l == (C0)) & hits;                        \
+    while (mk) {                                                                 \
+      const int c = __ffsll(mk) - 1;                                             \
+      mk &= mk - 1;                                                              \
+      const int ja = myrow - __builtin_amdgcn_readlane(r0l, c);                  \
+      reg_entry<C0>(acc, brec + c, vals[c * kRegPitch + lane], ja, J0);          \
+    }                                                                            \
+  }
+    TKBN_REG_GROUP(-5) TKBN_REG_GROUP(-4) TKBN_REG_GROUP(-3) TKBN_REG_GROUP(-2) TKBN_REG_GROUP(-1)
+    TKBN_REG_GROUP(0) TKBN_REG_GROUP(1) TKBN_REG_GROUP(2) TKBN_REG_GROUP(3) TKBN_REG_GROUP(4)
+    TKBN_REG_GROUP(5) TKBN_REG_GROUP(6) TKBN_REG_GROUP(7) TKBN_REG_GROUP(8) TKBN_REG_GROUP(9)
+    TKBN_REG_GROUP(10) TKBN_REG_GROUP(11) TKBN_REG_GROUP(12) TKBN_REG_GROUP(13) TKBN_REG_GROUP(14)
+    TKBN_REG_GROUP(15)
+#undef TKBN_REG_GROUP
+    __syncthreads();
+  }
+
+  // every owned cell inside the grid is written (split tiles were zeroed first)
+  if (lane < ncc) {
+    float2* dst_row = grid + (rbase + lane) * g.G;
 #pragma unroll
-      for (int j0 = 0; j0 < 6; ++j0) {
-        const int l0 = st.l0[j0];
-        if (l0 < 0) continue;
-        const float2 v0 = cmul_conj(v, st.w0[j0]);
-        const int rowcell = l0 * tg.lstride[0];
+    for (int h = 0; h < kRegRowsPerWave; ++h) {
+      const int k0 = o0 + myrow + h;
+      if (k0 < g.K[0]) {
 #pragma unroll
-        for (int j1 = 0; j1 < 6; ++j1) {
-          const int l1 = st.l1[j1];
-          if (l1 < 0) continue;
-          const float2 c = cmul_conj(v0, st.w1[j1]);
-          const int a = (rowcell + l1) * kCoilPitch + lane;
-          if constexpr (DET) {
-            atomicAdd(are + a, (unsigned long long)(long long)rintf(c.x * scale));
-            atomicAdd(aim + a, (unsigned long long)(long long)rintf(c.y * scale));
-          } else {
-            atomicAdd(are + a, c.x);
-            atomicAdd(aim + a, c.y);
+        for (int c = 0; c < kRegT1; ++c) {
+          const int k1 = o1 + c;
+          if (k1 < g.K[1]) {
+            float2* dst = dst_row + (int64_t)k0 * g.stride[0] + k1;
+            if (nsplit == 1) {
+              *dst = acc[h][c];
+            } else {
+              unsafeAtomicAdd(&dst->x, acc[h][c].x);
+              unsafeAtomicAdd(&dst->y, acc[h][c].y);
+            }
           }
         }
       }
-    }
-    __syncthreads();
-  }
-
-  // flush: cell fastest -> coalesced global stores along tile rows
-  const float inv = 1.f / scale;
-  for (int idx = threadIdx.x; idx < ncc * tcells; idx += blockDim.x) {
-    const int c = idx / tcells;
-    const int cell = idx - c * tcells;
-    const int l0 = cell / tg.lstride[0];
-    const int l1 = cell - l0 * tg.lstride[0];
-    const int k0 = o0 + l0, k1 = o1 + l1;
-    if (k0 >= g.K[0] || k1 >= g.K[1]) continue;
-    const int a = cell * kCoilPitch + c;
-    float2 val;
-    if constexpr (DET) {
-      val.x = (float)((double)(long long)are[a] * (double)inv);
-      val.y = (float)((double)(long long)aim[a] * (double)inv);
-    } else {
-      val.x = are[a];
-      val.y = aim[a];
-    }
-    float2* dst = grid + (r0 + c) * g.G + (int64_t)k0 * g.stride[0] + k1;
-    if (nsplit == 1) {
-      *dst = val;
-    } else {
-      unsafeAtomicAdd(&dst->x, val.x);
-      unsafeAtomicAdd(&dst->y, val.y);
     }
   }
 }
@@ -871,152 +1078,96 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
 }
 
 // ---- multi-coil (lanes = coil rows) variant -------------------------------------
-static int device_max_lds() {
-  static int cached[64];
-  static bool init[64];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
-  if (!init[dev]) {
-    int v = 65536;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-    cached[dev] = v;
-    init[dev] = true;
-  }
-  return cached[dev];
-}
-
-static TileGeom make_coil_tile_geom(const tkbn_interp_desc* d, bool det) {
-  // same plan layout as the sample-lane spread, with 16 x 16 (fast) / 16 x 8 (det) tiles
+static TileGeom make_coil_tile_geom(const tkbn_interp_desc* d) {
+  // same plan layout as the sample-lane spread, with 16 x 16 tiles
   tkbn_interp_desc dd = *d;
-  TileGeom tg = make_tile_geom(&dd, det);
-  const int T0 = 16, T1 = det ? 8 : 16;
-  const int TT[2] = {T0, T1};
-  int tcells = 1;
+  TileGeom tg = make_tile_geom(&dd, false);
+  const int TT[2] = {kRegT0, kRegT1};
   for (int i = 0; i < 2; ++i) {
-    int T = TT[i];
-    if (T > d->grid_size[i]) T = (int)d->grid_size[i];
-    tg.T[i] = T;
-    tg.nb[i] = (int)ceil_div(d->grid_size[i], T);
-    tcells *= T;
+    tg.T[i] = TT[i];
+    tg.nb[i] = (int)ceil_div(d->grid_size[i], TT[i]);
   }
   tg.bstride[1] = 1;
   tg.bstride[0] = tg.nb[1];
   tg.lstride[1] = 1;
   tg.lstride[0] = tg.T[1];
   tg.NB = tg.nb[0] * tg.nb[1];
-  tg.tcells = tcells;
+  tg.tcells = kRegT0 * kRegT1;
   tg.CC = 64;
-  tg.target = det ? 0x3fffffff : 1024;
+  tg.target = 1024;
+  // tiles a footprint can meet per dimension: 2, or 3 when the last tile is shorter
+  // than J - 1 (the footprint then crosses it into the first tile)
   int64_t per_sample = 1;
   for (int i = 0; i < 2; ++i) {
-    int mt = 1;
-    if (tg.nb[i] > 1) {
-      const int last = tg.K[i] - (tg.nb[i] - 1) * tg.T[i];
-      mt = last >= tg.J[i] - 1 ? 2 : 3;
-    }
-    per_sample *= mt;
+    const int last = tg.K[i] - (tg.nb[i] - 1) * tg.T[i];
+    per_sample *= last >= tg.J[i] - 1 ? 2 : 3;
   }
   const int64_t M = d->klength;
   tg.maxent = per_sample * M;
-  tg.maxwork = det ? tg.NB : tg.NB + tg.maxent / tg.target + 1;
+  tg.maxwork = tg.NB + tg.maxent / tg.target + 1;
+  tg.nkeys = (int64_t)tg.NB * kRegKeys;
   int64_t o = 0;
   tg.o_key = o;
   tg.o_perm = o; o += tg.maxent;
-  tg.o_count = o; o += tg.NB;
-  tg.o_start = o; o += tg.NB + 1;
-  tg.o_cursor = o; o += tg.NB;
+  tg.o_count = o; o += tg.nkeys;
+  tg.o_start = o; o += tg.nkeys + 1;
+  tg.o_cursor = o; o += tg.nkeys;
   tg.o_wstart = o; o += tg.NB + 1;
   tg.o_work = o; o += (tg.maxwork > tg.NB ? tg.maxwork : tg.NB);
   tg.o_nwork = o; o += 1;
-  o = (o + 3) & ~3LL;
+  o = (o + 3) & ~3LL;  // 16-byte aligned records
   tg.plan_ints = o;
+  tg.o_rec_bytes = o * 4;
   return tg;
 }
 
-static size_t coil_lds_bytes(const TileGeom& tg, bool det) {
-  return (size_t)2 * coil_plane_elems(tg.tcells) * (det ? 8 : 4) + 4 * kCoilBatch * sizeof(CoilStage);
+static int64_t coil_workspace_bytes(const TileGeom& tg) {
+  return tg.o_rec_bytes + tg.maxent * (int64_t)sizeof(CoilRec);
 }
 
+// register-accumulation spread: complex64, 2-D, shared trajectory, J <= 6, enough
+// coil rows to fill the lanes, and grids of at least tile + J - 1 cells per dimension
 static bool coil_variant_eligible(const tkbn_interp_desc* d) {
   if (d->dtype != TKBN_F32 || d->ndim != 2 || d->omega_batched) return false;
   if (d->numpoints[0] > 6 || d->numpoints[1] > 6) return false;
   if (d->batch * d->coils < 32) return false;
-  return d->grid_size[0] >= 8 && d->grid_size[1] >= 8;
+  return d->grid_size[0] >= kRegT0 + 5 && d->grid_size[1] >= kRegT1 + 5;
 }
 
-static bool coil_variant_usable(const tkbn_interp_desc* d, bool det) {
-  if (!coil_variant_eligible(d)) return false;
-  const TileGeom tg = make_coil_tile_geom(d, det);
-  return (int64_t)coil_lds_bytes(tg, det) <= device_max_lds();
-}
-
-template <bool DET>
-static int set_coil_lds(size_t bytes) {
-  static size_t done = 0;
-  if (bytes > done) {
-    TKBN_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&spread_coils_kernel<DET>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    done = bytes;
-  }
-  return TKBN_OK;
-}
-
-static int spread_coils_launch(const tkbn_interp_desc* d, bool det, const void* kdata, const void* omega, void* ws,
+static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, const void* omega, void* ws,
                                void* grid, hipStream_t s) {
   const Geom<float> g = make_geom<float>(d);
-  const TileGeom tg = make_coil_tile_geom(d, det);
+  const TileGeom tg = make_coil_tile_geom(d);
   int* wsi = static_cast<int*>(ws);
-  void* slot = static_cast<char*>(ws) + tg.plan_ints * 4;
   int st = launch_zero_fill<int>(wsi, tg.plan_ints, s);
   if (st != TKBN_OK) return st;
   dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
   if (g.M > 0) {
-    bin_count_kernel<float, 2><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
-    TKBN_LAUNCH_CHECK("bin_count_kernel<coils>");
+    coil_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
+    TKBN_LAUNCH_CHECK("coil_bin_count_kernel");
   }
-  plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi);
-  TKBN_LAUNCH_CHECK("plan_scan_kernel<coils>");
+  coil_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi);
+  TKBN_LAUNCH_CHECK("coil_plan_scan_kernel");
   if (g.M > 0) {
-    bin_scatter_kernel<float, 2><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
-    TKBN_LAUNCH_CHECK("bin_scatter_kernel<coils>");
+    coil_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
+    TKBN_LAUNCH_CHECK("coil_bin_scatter_kernel");
   }
   const int64_t nrows = g.B * g.C;
   const unsigned ychunks = (unsigned)ceil_div(nrows, 64);
-  const size_t lds = coil_lds_bytes(tg, det);
-  if (det) {
-    st = launch_zero_fill<unsigned long long>(static_cast<unsigned long long*>(slot), 1, s);
-    if (st != TKBN_OK) return st;
-    const int64_t n = nrows * g.M * 2;
-    if (n > 0) {
-      const int64_t blocks = ceil_div(n, 256) < 2048 ? ceil_div(n, 256) : 2048;
-      maxabs_kernel<float><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(static_cast<const float*>(kdata), n, slot);
-      TKBN_LAUNCH_CHECK("maxabs_kernel<coils>");
-    }
-    st = set_coil_lds<true>(lds);
-    if (st != TKBN_OK) return st;
-    spread_coils_kernel<true><<<dim3((unsigned)tg.maxwork, ychunks, 1), dim3(256), lds, s>>>(
-        g, tg, static_cast<const float2*>(kdata), static_cast<const float*>(omega), wsi, slot,
-        static_cast<float2*>(grid));
-    TKBN_LAUNCH_CHECK("spread_coils_kernel<det>");
-  } else {
-    zero_hot_tiles_kernel<float, 2><<<dim3((unsigned)tg.NB, (unsigned)ceil_div(nrows, tg.CC), 1), dim3(256), 0, s>>>(
-        g, tg, wsi, static_cast<float2*>(grid));
-    TKBN_LAUNCH_CHECK("zero_hot_tiles_kernel<coils>");
-    st = set_coil_lds<false>(lds);
-    if (st != TKBN_OK) return st;
-    spread_coils_kernel<false><<<dim3((unsigned)tg.maxwork, ychunks, 1), dim3(256), lds, s>>>(
-        g, tg, static_cast<const float2*>(kdata), static_cast<const float*>(omega), wsi, slot,
-        static_cast<float2*>(grid));
-    TKBN_LAUNCH_CHECK("spread_coils_kernel");
-  }
+  zero_hot_tiles_kernel<float, 2><<<dim3((unsigned)tg.NB, ychunks, 1), dim3(256), 0, s>>>(
+      g, tg, wsi, static_cast<float2*>(grid));
+  TKBN_LAUNCH_CHECK("zero_hot_tiles_kernel<coils>");
+  spread_coil_regs_kernel<<<dim3((unsigned)tg.maxwork, ychunks, 1), dim3(kRegThreads), 0, s>>>(
+      g, tg, static_cast<const float2*>(kdata), wsi, static_cast<float2*>(grid));
+  TKBN_LAUNCH_CHECK("spread_coil_regs_kernel");
   return TKBN_OK;
 }
 
 template <typename T>
 int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const void* omega, void* grid, void* ws,
                  hipStream_t s) {
-  if (std::is_same<T, float>::value && coil_variant_usable(d, det))
-    return spread_coils_launch(d, det, kdata, omega, ws, grid, s);
+  if (std::is_same<T, float>::value && !det && coil_variant_eligible(d))
+    return spread_coils_launch(d, kdata, omega, ws, grid, s);
   const Geom<T> g = make_geom<T>(d);
   const TileGeom tg = make_tile_geom(d, det);
   const size_t lds = spread_lds_bytes(d, tg, det);
@@ -1086,9 +1237,8 @@ int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mo
   const bool det = mode == TKBN_ADJ_DETERMINISTIC;
   const TileGeom tg = make_tile_geom(desc, det);
   int64_t n = spread_workspace_bytes(desc, tg);
-  if (coil_variant_eligible(desc)) {
-    const TileGeom tc = make_coil_tile_geom(desc, det);
-    const int64_t nc = tc.plan_ints * 4 + 16;
+  if (!det && coil_variant_eligible(desc)) {
+    const int64_t nc = coil_workspace_bytes(make_coil_tile_geom(desc));
     n = nc > n ? nc : n;
   }
   return n;
