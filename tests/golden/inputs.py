@@ -38,6 +38,8 @@ PLAN_CASES = [
     ((38,), (77,), 5, 333, 2.5),
     ((160, 160, 160), None, 6, 1024, 2.34),
     ((128, 192), None, 6, 1024, 2.34),
+    # odd J where om / gam rounds below J/2 - 1 on one table row (shifted sparse columns)
+    ((50, 45), (100, 90), (5, 3), 1024, 2.34),
 ]
 
 

@@ -2,7 +2,7 @@
 
 Usage (from the repository root, with the reference checkout at /root/reference):
 
-    PYTHONPATH=/root/reference python tests/golden/make_golden.py
+    PYTHONPATH=/root/reference python tests/golden/make_golden.py [fixtures] [c64] [plans] [grad]
 
 Why the reference is run here
 -----------------------------
@@ -107,10 +107,11 @@ def grad_cases():
 
 
 if __name__ == "__main__":
-    fixture_cases()
-    c64_cases()
-    plan_cases()
-    grad_cases()
+    # optional arguments select parts: fixtures c64 plans grad (default: all)
+    parts = sys.argv[1:] or ["fixtures", "c64", "plans", "grad"]
+    for name, fn in (("fixtures", fixture_cases), ("c64", c64_cases), ("plans", plan_cases), ("grad", grad_cases)):
+        if name in parts:
+            fn()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

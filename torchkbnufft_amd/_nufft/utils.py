@@ -43,8 +43,18 @@ def _table_1d(im_size: int, grid_size: int, numpoints: int, table_oversamp: int,
     weight = np.real(weight)
     phase = np.exp(1j * gam * (im_size - 1) / 2 * kern)
     coef = np.conj(phase * weight) * np.exp(1j * 0.0)
-    # neighbour j = J first (column J-1), down to j = 1, then the closing zero
-    return np.concatenate([coef[:, ::-1].T.reshape(-1), np.array([0])])
+    # Tap j of row k is stored in grid column (j + koff[k]) mod K, as the reference's
+    # sparse matrix does (utils.py:83-85), and only columns 0 .. J-1 form the table
+    # (utils.py:169-172; duplicates add).  koff is -1 except on rows where om / gam
+    # rounds below J/2 - 1: there every tap moves one column to the right.
+    koff = np.floor(om / gam - numpoints / 2)
+    cols = np.mod(np.arange(1, numpoints + 1)[None, :] + koff[:, None], grid_size).astype(np.int64)
+    tab = np.zeros((table_oversamp, numpoints), dtype=np.complex128)
+    for j in range(numpoints):
+        rows = np.nonzero(cols[:, j] < numpoints)[0]
+        np.add.at(tab, (rows, cols[rows, j]), coef[rows, j])
+    # column J-1 first, down to column 0, then the closing zero
+    return np.concatenate([tab[:, ::-1].T.reshape(-1), np.array([0])])
 
 
 def build_table(

@@ -828,6 +828,200 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
 }
 
 // ---------------------------------------------------------------------------
+// multi-coil forward gather (2-D, complex64, J <= 6)
+//   The plan bins every sample once, into the tile of its base cell (the
+//   footprint's first cell), keeping trajectory order inside a tile, and stores a
+//   per-entry record: phase * w0[j], w1[j] and the base cell relative to the tile.
+//   One group (8 waves) per (tile, split, chunk of 64 coil rows) stages the tile and
+//   its J-1 halo (wrapped) in LDS with the coil row fastest, so a tap read by the 64
+//   lanes (= coil rows) of a wave is one conflict-free LDS access.  Each wave takes
+//   one entry at a time; its weights are read at a wave-uniform address.
+// ---------------------------------------------------------------------------
+constexpr int kGatT0 = 4, kGatT1 = 8;       // tile edges (the region fits twice per CU)
+constexpr int kGatR0 = kGatT0 + 5;          // staged region edges (J <= 6)
+constexpr int kGatR1 = kGatT1 + 5;
+constexpr int kGatCells = kGatR0 * kGatR1;
+constexpr int kGatPitch = 65;               // float2 per staged cell: 64 rows + 1
+constexpr int kGatThreads = 512;
+constexpr int kGatTarget = 256;             // entries per work item
+
+struct GatRec {
+  float2 w0[6];  // phase * w0[j] along dimension 0
+  float2 w1[6];  // w1[j] along dimension 1
+  int l0, l1;    // base cell relative to the tile origin
+  int m;         // sample index
+  int pad;
+};
+static_assert(sizeof(GatRec) == 112, "plan record is 112 bytes");
+
+__device__ __forceinline__ int gat_base_tile(const Geom<float>& g, const TileGeom& tg, const float* om, int m, int* l) {
+  int key = 0;
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int T = d == 0 ? kGatT0 : kGatT1;
+    const int bm = (int)sample_base_wrapped<float>(g, d, om[d * g.M + m]);
+    const int tile = bm / T;
+    l[d] = bm - tile * T;
+    key += tile * tg.bstride[d];
+  }
+  return key;
+}
+
+__global__ void __launch_bounds__(256)
+    gat_bin_count_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega, int* __restrict__ ws) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = m < g.M;
+  int l[2];
+  const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
+  wave_aggregated_add(ws + tg.o_count, key, live);
+}
+
+__global__ void __launch_bounds__(256)
+    gat_bin_scatter_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
+                           int* __restrict__ ws) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = m < g.M;
+  int l[2] = {0, 0};
+  const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
+  const int pos = wave_aggregated_inc(ws + tg.o_cursor, key, live);
+  if (!live) return;
+  GatRec rec;
+  float2 w[6];
+  int mdx[6];
+  const float theta = sample_theta<float, 2>(g, omega, m);
+  const float2 ph = make_float2(cosf(theta), sinf(theta));
+  sample_taps_md<float, 6>(g, 0, omega[m], w, mdx);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) rec.w0[j] = cmul(ph, w[j]);
+  sample_taps_md<float, 6>(g, 1, omega[g.M + m], w, mdx);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) rec.w1[j] = w[j];
+  rec.l0 = l[0];
+  rec.l1 = l[1];
+  rec.m = m;
+  rec.pad = 0;
+  ws[tg.o_perm + pos] = m;
+  reinterpret_cast<GatRec*>(reinterpret_cast<char*>(ws) + tg.o_rec_bytes)[pos] = rec;
+}
+
+// Work item of group b in a grid of n groups, n a multiple of 8: consecutive groups go
+// to different XCDs (round robin), so item = (b mod 8) * (n / 8) + b / 8 gives each
+// XCD a contiguous run of tiles and neighbouring tiles share their halo through that
+// XCD's L2.  The map is a bijection of [0, n) because n is a multiple of 8.
+__device__ __forceinline__ int64_t xcd_work_item(int64_t b, int64_t n) {
+  return (b % 8) * (n / 8) + b / 8;
+}
+
+__global__ void __launch_bounds__(kGatThreads)
+    gather_coils_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ grid,
+                        const int* ws, float2* kdata) {  // no restrict: keeps the record loads vector loads
+  __shared__ __attribute__((aligned(16))) float2 region[kGatCells * kGatPitch];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int* plan = ws;
+  const int64_t wi = xcd_work_item(blockIdx.x, gridDim.x);
+  if (wi >= plan[tg.o_nwork]) return;
+  const int* wstart = plan + tg.o_wstart;
+  const int t = plan[tg.o_work + wi];
+  const int split = (int)(wi - wstart[t]);
+  const int V = plan[tg.o_count + t];
+  const int i0 = split * tg.target;
+  const int i1 = min(i0 + tg.target, V);
+  if (i0 >= i1) return;  // no sample is based in this tile
+  const int64_t nrows = g.B * g.C;
+  const int64_t rbase = (int64_t)blockIdx.y * 64;
+  const int ncc = (int)((rbase + 64 < nrows ? rbase + 64 : nrows) - rbase);
+  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kGatT0;
+  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kGatT1;
+
+  // stage region[cell][row] = grid[row][(o + l) mod K], cell = l0 * kGatR1 + l1.  A
+  // thread owns one cell and every third row: all its loads are issued before any
+  // LDS store, so the staging costs one memory latency.
+  constexpr int kGroups = kGatThreads / kGatCells;          // 3 row groups
+  constexpr int kPer = (64 + kGroups - 1) / kGroups;        // 22 rows per thread
+  const int cell = threadIdx.x % kGatCells, cg = threadIdx.x / kGatCells;
+  int64_t off = 0;
+  {
+    const int l0 = cell / kGatR1, l1 = cell - l0 * kGatR1;
+    int k0 = o0 + l0, k1 = o1 + l1;
+    while (k0 >= g.K[0]) k0 -= g.K[0];
+    while (k1 >= g.K[1]) k1 -= g.K[1];
+    off = (int64_t)k0 * g.stride[0] + k1;
+  }
+  float2 v[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int c = cg + q * kGroups;
+    v[q] = make_float2(0.f, 0.f);
+    if (cg < kGroups && c < ncc) v[q] = grid[(rbase + c) * g.G + off];
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int c = cg + q * kGroups;
+    if (cg < kGroups && c < ncc) region[cell * kGatPitch + c] = v[q];
+  }
+  __syncthreads();
+
+  // Records are read with vector loads (uniform address), one entry ahead of use: the
+  // LDS reads then never wait on them (scalar loads would share the LDS wait counter).
+  const float4* recs = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) +
+                       (int64_t)plan[tg.o_start + t] * 7;
+  const int J0 = g.J[0], J1 = g.J[1];
+  constexpr int kStep = kGatThreads / 64;
+  float4 cur[7], nxt[7];
+  int e = i0 + wave;
+  if (e < i1) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) cur[q] = recs[(int64_t)e * 7 + q];
+  }
+  for (; e < i1; e += kStep) {
+    if (e + kStep < i1) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q) nxt[q] = recs[(int64_t)(e + kStep) * 7 + q];
+    }
+    // record words: w0 in cur[0..2], w1 in cur[3..5], (l0, l1, m, pad) in cur[6]
+    float2 w0[6], w1[6];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      w0[2 * q] = make_float2(cur[q].x, cur[q].y);
+      w0[2 * q + 1] = make_float2(cur[q].z, cur[q].w);
+      w1[2 * q] = make_float2(cur[3 + q].x, cur[3 + q].y);
+      w1[2 * q + 1] = make_float2(cur[3 + q].z, cur[3 + q].w);
+    }
+    const int l0 = __builtin_amdgcn_readfirstlane(__float_as_int(cur[6].x));
+    const int l1 = __builtin_amdgcn_readfirstlane(__float_as_int(cur[6].y));
+    const int m = __builtin_amdgcn_readfirstlane(__float_as_int(cur[6].z));
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j0 = 0; j0 < 6; ++j0) {
+      if (j0 < J0) {
+        const float2* L = region + ((l0 + j0) * kGatR1 + l1) * kGatPitch + lane;
+        float2 p = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j1 = 0; j1 < 6; ++j1) {
+          if (j1 < J1) {
+            const float2 w = w1[j1];
+            const float2 v = L[j1 * kGatPitch];
+            p.x = fmaf(w.x, v.x, p.x);
+            p.x = fmaf(-w.y, v.y, p.x);
+            p.y = fmaf(w.x, v.y, p.y);
+            p.y = fmaf(w.y, v.x, p.y);
+          }
+        }
+        const float2 w = w0[j0];
+        acc.x = fmaf(w.x, p.x, acc.x);
+        acc.x = fmaf(-w.y, p.y, acc.x);
+        acc.y = fmaf(w.x, p.y, acc.y);
+        acc.y = fmaf(w.y, p.x, acc.y);
+      }
+    }
+    if (lane < ncc) kdata[(rbase + lane) * g.M + m] = acc;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) cur[q] = nxt[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // forward gather from an LDS-staged tile (standard grid layout)
 //   one group per (tile, split, coil chunk): stage the tile plus its J-1 halo cells
 //   (wrapped) for the chunk's rows, then every sample based in the tile reads its
@@ -1212,7 +1406,76 @@ int interp_fwd_tiled_t(const tkbn_interp_desc* d, const void* grid, const void* 
   TKBN_DISPATCH_DJ(d->ndim, max_numpoints(d), fwd_tiled_launch, g, tg, grid, omega, wsi, kdata, s);
 }
 
+// ---- multi-coil forward gather ---------------------------------------------------
+static bool gather_coils_eligible(const tkbn_interp_desc* d) {
+  if (d->dtype != TKBN_F32 || d->ndim != 2 || d->omega_batched) return false;
+  if (d->numpoints[0] > 6 || d->numpoints[1] > 6) return false;
+  return d->batch * d->coils >= 32;
+}
+
+static TileGeom make_gather_geom(const tkbn_interp_desc* d) {
+  TileGeom tg{};
+  const int TT[2] = {kGatT0, kGatT1};
+  for (int i = 0; i < 2; ++i) {
+    tg.K[i] = (int)d->grid_size[i];
+    tg.J[i] = (int)d->numpoints[i];
+    tg.T[i] = TT[i];
+    tg.nb[i] = (int)ceil_div(d->grid_size[i], TT[i]);
+  }
+  tg.K[2] = tg.J[2] = tg.T[2] = tg.nb[2] = 1;
+  tg.bstride[1] = 1;
+  tg.bstride[0] = tg.nb[1];
+  tg.bstride[2] = 0;
+  tg.NB = tg.nb[0] * tg.nb[1];
+  tg.own_only = 1;
+  tg.CC = 64;
+  tg.target = kGatTarget;
+  const int64_t M = d->klength;
+  tg.maxent = M;
+  tg.maxwork = tg.NB + M / tg.target + 1;
+  int64_t o = 0;
+  tg.o_key = o;
+  tg.o_perm = o; o += tg.maxent;
+  tg.o_count = o; o += tg.NB;
+  tg.o_start = o; o += tg.NB + 1;
+  tg.o_cursor = o; o += tg.NB;
+  tg.o_wstart = o; o += tg.NB + 1;
+  tg.o_work = o; o += (tg.maxwork > tg.NB ? tg.maxwork : tg.NB);
+  tg.o_nwork = o; o += 1;
+  o = (o + 3) & ~3LL;
+  tg.plan_ints = o;
+  tg.o_rec_bytes = o * 4;
+  return tg;
+}
+
+static int64_t gather_workspace_bytes(const TileGeom& tg) {
+  return tg.o_rec_bytes + tg.maxent * (int64_t)sizeof(GatRec);
+}
+
+static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, const void* omega, void* kdata, void* ws,
+                               hipStream_t s) {
+  const Geom<float> g = make_geom<float>(d);
+  const TileGeom tg = make_gather_geom(d);
+  int* wsi = static_cast<int*>(ws);
+  int st = launch_zero_fill<int>(wsi, tg.plan_ints, s);
+  if (st != TKBN_OK) return st;
+  dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
+  gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
+  TKBN_LAUNCH_CHECK("gat_bin_count_kernel");
+  plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi);
+  TKBN_LAUNCH_CHECK("plan_scan_kernel<gather>");
+  gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
+  TKBN_LAUNCH_CHECK("gat_bin_scatter_kernel");
+  const unsigned ychunks = (unsigned)ceil_div(g.B * g.C, 64);
+  const int64_t groups = ceil_div(tg.maxwork, 8) * 8;  // a multiple of 8 (xcd_work_item)
+  gather_coils_kernel<<<dim3((unsigned)groups, ychunks, 1), dim3(kGatThreads), 0, s>>>(
+      g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata));
+  TKBN_LAUNCH_CHECK("gather_coils_kernel");
+  return TKBN_OK;
+}
+
 int64_t interp_fwd_workspace_bytes(const tkbn_interp_desc* d) {
+  if (gather_coils_eligible(d)) return gather_workspace_bytes(make_gather_geom(d));
   const TileGeom tg = make_tile_geom(d, false, true);
   const size_t lds = (size_t)tg.CC * tg.rcells * (d->dtype == TKBN_F32 ? 8 : 16);
   if (lds > (size_t)kLdsBudget) return 0;  // region too large for LDS: plain kernel
@@ -1221,6 +1484,7 @@ int64_t interp_fwd_workspace_bytes(const tkbn_interp_desc* d) {
 
 int interp_fwd_tiled(const tkbn_interp_desc* d, const void* grid, const void* omega, void* kdata, void* ws,
                      hipStream_t s) {
+  if (gather_coils_eligible(d)) return gather_coils_launch(d, grid, omega, kdata, ws, s);
   if (d->dtype == TKBN_F32) return interp_fwd_tiled_t<float>(d, grid, omega, kdata, ws, s);
   return interp_fwd_tiled_t<double>(d, grid, omega, kdata, ws, s);
 }
