@@ -77,27 +77,34 @@ def stage_bytes(F: int, M: int) -> dict:
     }
 
 
-def time_stages(ops, x, ktraj, smaps, iters: int) -> dict:
-    """Average duration of each pipeline stage (the same calls the modules make),
-    HIP events on the current stream, which is the stream the kernels run on."""
+def time_stages(ops, x, ktraj, smaps, iters: int):
+    """Average duration of each pipeline stage (the same calls the modules make) and of
+    the two binned interpolation kernels, from HIP events on the current stream, which
+    is the stream the kernels run on (the library records the kernel events itself,
+    tkbn_profile_next_kernel)."""
+    from torchkbnufft_amd import _lib
     from torchkbnufft_amd._nufft import fft as _fft
     from torchkbnufft_amd._nufft import interp as _interp
 
     fw, ad = ops
     tabs = fw._tables()
+    lib = _lib.load()
     fused = _fft.fused_supported(x.dtype, (N, N), (K, K))
     names = ["fft_fwd", "interp", "spread", "fft_adj"]
-    acc = {n: 0.0 for n in names}
+    kern = ["gather_coils_kernel", "spread_coil_regs_kernel"]
+    acc = {n: 0.0 for n in names + kern}
 
-    def run(ev):
+    def run(ev, kev):
         ev[0].record()
         if fused:
             g = _fft.fft_grid_fwd(x, smaps, fw.scaling_coef, (N, N), (K, K), 1.0)
         else:
             g = torch.fft.fftn(_fft.apod_pad(x, smaps, fw.scaling_coef, (N, N), (K, K), 1.0), dim=(-2, -1))
         ev[1].record()
+        lib.tkbn_profile_next_kernel(kev[0].cuda_event, kev[1].cuda_event)
         kd = _interp.table_interp(g, ktraj, tabs, fw.n_shift, fw.numpoints, fw.table_oversamp, fw.offsets)
         ev[2].record()
+        lib.tkbn_profile_next_kernel(kev[2].cuda_event, kev[3].cuda_event)
         g2 = _interp.table_interp_adjoint(kd, ktraj, tabs, fw.n_shift, fw.numpoints, fw.table_oversamp, fw.offsets, (K, K))
         ev[3].record()
         if fused:
@@ -106,15 +113,25 @@ def time_stages(ops, x, ktraj, smaps, iters: int) -> dict:
             _fft.crop_deapod(torch.fft.ifftn(g2, dim=(-2, -1), norm="forward"), smaps, fw.scaling_coef, (N, N), 1.0)
         ev[4].record()
 
-    run([torch.cuda.Event(enable_timing=True) for _ in range(5)])  # warm (plans, caches)
+    def events(n):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+        for e in ev:
+            e.record()  # torch creates the HIP event on first record; the library re-records it
+        return ev
+
+    run(events(5), events(4))  # warm (plans, caches)
     torch.cuda.synchronize()
     for _ in range(iters):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
-        run(ev)
+        ev, kev = events(5), events(4)
+        run(ev, kev)
         torch.cuda.synchronize()
         for i, n in enumerate(names):
             acc[n] += ev[i].elapsed_time(ev[i + 1]) * 1e3
-    return {n: acc[n] / iters for n in names}
+        acc[kern[0]] += kev[0].elapsed_time(kev[1]) * 1e3
+        acc[kern[1]] += kev[2].elapsed_time(kev[3]) * 1e3
+    stages = {n: acc[n] / iters for n in names}
+    kernels = {n: acc[n] / iters for n in kern}
+    return stages, kernels
 
 
 def cpu_baseline(x, ktraj, smaps, seconds: float) -> dict:
@@ -206,9 +223,11 @@ def main() -> int:
 
     F = footprint_cells(ktraj_cpu.numpy(), (K, K), J)
     sb = stage_bytes(F, M)
-    stages = time_stages((fw, ad), x, ktraj, smaps, args.stage_iters)
-    dom = max(stages, key=stages.get)
-    achieved = sb[dom] / (stages[dom] * 1e-6) / 1e9
+    stages, kernels = time_stages((fw, ad), x, ktraj, smaps, args.stage_iters)
+    # algorithmic bytes of one launch of each binned kernel = its stage's bytes
+    kbytes = {"gather_coils_kernel": sb["interp"], "spread_coil_regs_kernel": sb["spread"]}
+    dom = max(kernels, key=kernels.get)
+    achieved = kbytes[dom] / (kernels[dom] * 1e-6) / 1e9
     pair_bytes = sum(sb.values())
 
     out = {
@@ -236,6 +255,8 @@ def main() -> int:
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
+            "algorithmic_bytes": kbytes[dom],
+            "avg_us": round(kernels[dom], 2),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -248,6 +269,7 @@ def main() -> int:
             "footprint_frac": F / (K * K),
         },
         "stages_us": {k: round(v, 2) for k, v in stages.items()},
+        "kernels_us": {k: round(v, 2) for k, v in kernels.items()},
         "stage_bytes": sb,
         "fp64_oracle_rel_err": fp64_oracle_error(fw, x, ktraj, smaps),
     }

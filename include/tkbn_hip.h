@@ -12,6 +12,14 @@
  *   - returns TKBN_OK (0) or an error code; tkbn_last_error() then returns a
  *     thread-local message.
  *
+ * Interpolation workspaces (tkbn_interp_fwd / tkbn_interp_adj) start with a 256-byte
+ * plan-cache header.  Zero those 256 bytes before a workspace's first use; then pass
+ * the same workspace to later calls unchanged: when omega, the tables and the
+ * geometry are bitwise those of the previous call with that workspace (checked on
+ * the device, on the stream), the trajectory plan (bins, permutation, weight
+ * records) is reused instead of rebuilt.  Calls that share a workspace must be
+ * ordered (same stream).
+ *
  * Reference interfaces replaced (torchkbnufft, file:line):
  *   tkbn_interp_fwd      <- _nufft/interp.py:315-403  table_interp
  *                           (+ table_interp_one_batch :154-203, calc_coef_and_indices :89-150)
@@ -28,6 +36,8 @@
  *   tkbn_interp_domega   <- no reference counterpart: the reference's autograd Functions
  *                           return None for the trajectory (_autograd/interp.py:130,178);
  *                           this gives d/d omega for both of them.
+ *   tkbn_profile_next_kernel <- no reference counterpart: measurement hook (HIP events
+ *                           around the next binned interpolation kernel).
  */
 #ifndef TKBN_HIP_H
 #define TKBN_HIP_H
@@ -38,7 +48,7 @@
 extern "C" {
 #endif
 
-#define TKBN_ABI_VERSION 2
+#define TKBN_ABI_VERSION 3
 
 #if defined(__GNUC__)
 #define TKBN_API __attribute__((visibility("default")))
@@ -103,7 +113,15 @@ TKBN_API int64_t tkbn_interp_fwd_workspace_bytes(const tkbn_interp_desc* desc);
 TKBN_API int tkbn_interp_fwd(const tkbn_interp_desc* desc, const void* grid, const void* omega, void* kdata,
                              void* workspace, int64_t workspace_bytes, void* stream);
 
-/* Bytes of device scratch tkbn_interp_adj needs for `mode` (0 when none). */
+/*
+ * Measurement hook: the next binned interpolation kernel that this thread enqueues
+ * (the gather of tkbn_interp_fwd or the spread of tkbn_interp_adj, on their multi-coil
+ * paths) is bracketed by hipEventRecord(start_event) / hipEventRecord(stop_event) on
+ * its stream.  One-shot; either event may be NULL.
+ */
+TKBN_API int tkbn_profile_next_kernel(void* start_event, void* stop_event);
+
+/* Bytes of device workspace tkbn_interp_adj needs for `mode` (see the header note). */
 TKBN_API int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mode);
 
 /* grid = table_interp_adjoint(kdata, omega); `grid` is fully overwritten. */

@@ -274,3 +274,41 @@ def test_multicoil_forward_matches_oracle(grid, rows, M, J):
     finally:
         _interp.set_binned_forward_threshold(old)
     assert rel_l2(out, plain) < 1e-6
+
+
+def test_plan_cache_reuse_and_invalidation():
+    """The trajectory plan kept in a reused workspace must never go stale."""
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    print("[case] plan cache reuse / invalidation", flush=True)
+    gen = torch.Generator().manual_seed(41)
+    grid, rows, M = (96, 96), 40, 30000
+    om = create_ktraj(2, M, gen, torch.float32).to(DEV)
+    x = gi.cplx((1, rows) + grid, gen).to(DEV)
+    y = gi.cplx((1, rows, M), gen).to(DEV)
+    fw = tkbn.KbInterp(im_size=(48, 48), grid_size=grid, dtype=torch.complex64).to(DEV)
+    ad = tkbn.KbInterpAdjoint(im_size=(48, 48), grid_size=grid, dtype=torch.complex64).to(DEV)
+
+    def fresh(fn, *a):
+        _interp.clear_plan_cache()
+        return fn(*a)
+
+    f1, a1 = fw(x, om), ad(y, om)
+    assert torch.equal(f1, fw(x, om)) and rel_l2(a1, ad(y, om)) < 1e-6  # reused plan
+    # in-place edit that torch does not version (through .data): must be seen
+    om.data[0, 5] += 0.25
+    om.data[1, M - 3] -= 0.5
+    f2, a2 = fw(x, om), ad(y, om)
+    assert torch.equal(f2, fresh(fw, x, om))
+    assert rel_l2(a2, fresh(ad, y, om)) < 1e-6
+    assert not torch.equal(f1, f2)
+    # a deterministic call through the same adjoint workspace, then the fast path again
+    d = _interp.table_interp_adjoint(y, om, ad._tables(), ad.n_shift, ad.numpoints, ad.table_oversamp, ad.offsets, grid,
+                                     deterministic=True)
+    assert rel_l2(ad(y, om), d) < 1e-6
+    # different table (numpoints) through the same trajectory tensor
+    fw4 = tkbn.KbInterp(im_size=(48, 48), grid_size=grid, numpoints=4, dtype=torch.complex64).to(DEV)
+    plan4 = tp.Plan((48, 48), grid, numpoints=4, dtype=torch.complex64)
+    ref4 = tp.kb_table_interp(plan4, x[:, :2].cpu(), om.cpu())
+    assert rel_l2(fw4(x, om)[:, :2].cpu(), ref4) < C64_TOL
+    assert torch.equal(fw(x, om), f2)

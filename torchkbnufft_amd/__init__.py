@@ -11,7 +11,7 @@ import torch  # noqa: F401  (load torch's HIP runtime before libtkbn_hip.so)
 from . import functional, modules
 from ._math import absolute, complex_mult, complex_sign, conj_complex_mult, imag_exp, inner_product
 from ._nufft import utils as nufft_utils
-from ._nufft.interp import deterministic_enabled, set_deterministic
+from ._nufft.interp import clear_plan_cache, deterministic_enabled, set_deterministic
 from .modules import KbInterp, KbInterpAdjoint, KbNufft, KbNufftAdjoint
 
 __version__ = "0.1.0"
@@ -32,4 +32,5 @@ __all__ = [
     "modules",
     "nufft_utils",
     "set_deterministic",
+    "clear_plan_cache",
 ]

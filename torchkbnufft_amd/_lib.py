@@ -63,6 +63,7 @@ _SIGNATURES = {
         [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_i32, _c_vp, _c_i64, _c_vp],
     ),
     "tkbn_interp_domega": (ctypes.c_int, [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "tkbn_profile_next_kernel": (ctypes.c_int, [_c_vp, _c_vp]),
     "tkbn_apod_pad": (
         ctypes.c_int,
         [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64, _c_i64,
@@ -87,7 +88,10 @@ _SIGNATURES = {
     ),
 }
 
-ABI_VERSION = 2
+ABI_VERSION = 3
+
+# bytes at the head of an interpolation workspace that hold the plan-cache header
+PLAN_CACHE_HEADER_BYTES = 256
 
 _lock = threading.Lock()
 _lib: Optional[ctypes.CDLL] = None

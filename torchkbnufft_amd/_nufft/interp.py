@@ -8,6 +8,7 @@ iterations of ATen gathers and ``index_add_`` calls.
 from __future__ import annotations
 
 import ctypes
+from collections import OrderedDict
 from typing import List, Optional, Sequence
 
 import torch
@@ -20,6 +21,33 @@ from ._host import host_values, real_dtype_of, require_gpu
 _DEFAULT_ADJ_MODE = _lib.TKBN_ADJ_ATOMIC
 # samples x rows from which the forward uses the binned, LDS-staged gather
 _BINNED_FWD_MIN_WORK = 1 << 16
+
+
+# Interpolation workspaces kept between calls: the library stores the trajectory plan
+# in them and reuses it while omega, the tables and the geometry stay bitwise equal
+# (checked on the device).  Keyed by (kind, device, stream, trajectory tensor); the
+# key only picks a buffer, correctness never depends on it.
+_WORKSPACES: "OrderedDict[tuple, torch.Tensor]" = OrderedDict()
+_WORKSPACES_MAX = 4
+
+
+def _plan_workspace(kind: str, omega: Tensor, like: Tensor, nbytes: int) -> Tensor:
+    key = (kind, like.device, _lib.stream_of(like), omega.data_ptr(), tuple(omega.shape), omega.dtype)
+    ws = _WORKSPACES.get(key)
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.empty(max(nbytes, _lib.PLAN_CACHE_HEADER_BYTES), dtype=torch.uint8, device=like.device)
+        ws[: _lib.PLAN_CACHE_HEADER_BYTES].zero_()  # a zero header never matches: first call builds
+        _WORKSPACES[key] = ws
+        while len(_WORKSPACES) > _WORKSPACES_MAX:
+            _WORKSPACES.popitem(last=False)
+    else:
+        _WORKSPACES.move_to_end(key)
+    return ws
+
+
+def clear_plan_cache() -> None:
+    """Release the workspaces that hold cached trajectory plans."""
+    _WORKSPACES.clear()
 
 
 def set_binned_forward_threshold(work: int) -> None:
@@ -110,6 +138,7 @@ def table_interp(
     """Interpolate gridded ``image`` (B, C, *K) to ``omega``; returns (B, C, M)."""
     omega = _check_omega(omega, image.shape[0], "image")
     require_gpu(image, omega)
+    key_omega = omega
     image = image.contiguous()
     tables = _prep_tables(tables, image)
     omega = _prep_omega(omega, image)
@@ -123,11 +152,11 @@ def table_interp(
     if omega.shape[-1] * B * C >= _BINNED_FWD_MIN_WORK:
         nws = int(lib.tkbn_interp_fwd_workspace_bytes(ctypes.byref(desc)))
         if nws > 0:
-            ws = torch.empty(nws, dtype=torch.uint8, device=image.device)
+            ws = _plan_workspace("fwd", key_omega, image, nws)
     with torch.cuda.device(image.device):
         st = lib.tkbn_interp_fwd(
             ctypes.byref(desc), image.data_ptr(), omega.data_ptr(), kdata.data_ptr(),
-            ws.data_ptr() if ws is not None else None, nws if ws is not None else 0, _lib.stream_of(image),
+            ws.data_ptr() if ws is not None else None, ws.numel() if ws is not None else 0, _lib.stream_of(image),
         )
     _lib.check(st, "tkbn_interp_fwd")
     return kdata
@@ -147,6 +176,7 @@ def table_interp_adjoint(
     """Spread off-grid ``data`` (B, C, M) onto a grid of ``grid_size``; returns (B, C, *K)."""
     omega = _check_omega(omega, data.shape[0], "data")
     require_gpu(data, omega)
+    key_omega = omega
     data = data.contiguous()
     tables = _prep_tables(tables, data)
     omega = _prep_omega(omega, data)
@@ -163,7 +193,7 @@ def table_interp_adjoint(
     ws_bytes = lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(desc), mode)
     if ws_bytes < 0:
         _lib.check(_lib.TKBN_ERR_INVALID, "tkbn_interp_adj_workspace_bytes")
-    ws = torch.empty(max(int(ws_bytes), 1), dtype=torch.uint8, device=data.device)
+    ws = _plan_workspace("adj", key_omega, data, int(ws_bytes))
     with torch.cuda.device(data.device):
         st = lib.tkbn_interp_adj(
             ctypes.byref(desc),
@@ -172,7 +202,7 @@ def table_interp_adjoint(
             grid.data_ptr(),
             mode,
             ws.data_ptr(),
-            int(ws_bytes),
+            ws.numel(),
             _lib.stream_of(data),
         )
     _lib.check(st, "tkbn_interp_adj")

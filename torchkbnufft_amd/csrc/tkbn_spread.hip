@@ -26,6 +26,7 @@
 #include <type_traits>
 
 #include "tkbn_geom.h"
+#include "tkbn_plancache.h"
 
 namespace tkbn {
 
@@ -225,7 +226,9 @@ __device__ void block_exclusive_scan(const int* in, int* out, int n, int* sbuf) 
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(1024) plan_scan_kernel(const TileGeom tg, int* __restrict__ ws) {
+__global__ void __launch_bounds__(1024) plan_scan_kernel(const TileGeom tg, int* __restrict__ ws,
+                                                        const CacheHdr* cache = nullptr) {
+  if (cache && cache->ok) return;  // cached plan reused
   __shared__ int sbuf[1024];
   int* plan = ws + (int64_t)blockIdx.x * tg.plan_ints;
   const int* count = plan + tg.o_count;
@@ -575,7 +578,8 @@ __device__ __forceinline__ void coil_sample_tiles(const Geom<float>& g, const Ti
 
 __global__ void __launch_bounds__(256)
     coil_bin_count_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
-                          int* __restrict__ ws) {
+                          int* __restrict__ ws, const CacheHdr* cache) {
+  if (cache->ok) return;  // cached plan reused
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   int tl[2][3], off[2][3], n[2];
@@ -587,7 +591,8 @@ __global__ void __launch_bounds__(256)
 
 __global__ void __launch_bounds__(256)
     coil_bin_scatter_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
-                            int* __restrict__ ws) {
+                            int* __restrict__ ws, const CacheHdr* cache) {
+  if (cache->ok) return;  // cached plan reused
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   int tl[2][3], off[2][3], n[2];
@@ -624,7 +629,9 @@ __global__ void __launch_bounds__(256)
 
 // Scan of the (tile, c0) counts: per-tile totals are scanned across the block, then
 // each tile's keys are offset locally.  The totals also give the splits and work list.
-__global__ void __launch_bounds__(1024) coil_plan_scan_kernel(const TileGeom tg, int* __restrict__ ws) {
+__global__ void __launch_bounds__(1024) coil_plan_scan_kernel(const TileGeom tg, int* __restrict__ ws,
+                                                             const CacheHdr* cache) {
+  if (cache->ok) return;  // cached plan reused
   __shared__ int sbuf[1024];
   int* plan = ws;
   const int* count = plan + tg.o_count;
@@ -868,7 +875,9 @@ __device__ __forceinline__ int gat_base_tile(const Geom<float>& g, const TileGeo
 }
 
 __global__ void __launch_bounds__(256)
-    gat_bin_count_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega, int* __restrict__ ws) {
+    gat_bin_count_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega, int* __restrict__ ws,
+                         const CacheHdr* cache) {
+  if (cache->ok) return;  // cached plan reused
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   int l[2];
@@ -878,7 +887,9 @@ __global__ void __launch_bounds__(256)
 
 __global__ void __launch_bounds__(256)
     gat_bin_scatter_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
-                           int* __restrict__ ws) {
+                           int* __restrict__ ws,
+                         const CacheHdr* cache) {
+  if (cache->ok) return;  // cached plan reused
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   int l[2] = {0, 0};
@@ -1328,33 +1339,65 @@ static bool coil_variant_eligible(const tkbn_interp_desc* d) {
   return d->grid_size[0] >= kRegT0 + 5 && d->grid_size[1] >= kRegT1 + 5;
 }
 
+// plan kinds (part of the cache key)
+constexpr int kPlanSpreadCoils = 1;
+constexpr int kPlanGatherCoils = 2;
+
+// One-shot timing hook: events the caller asked to have recorded around the next
+// main interpolation kernel (tkbn_profile_next_kernel).
+struct ProfileHook {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+static thread_local ProfileHook g_profile;
+
+static void profile_before(hipStream_t s) {
+  if (g_profile.start) hipEventRecord(g_profile.start, s);
+}
+static void profile_after(hipStream_t s) {
+  if (g_profile.stop) hipEventRecord(g_profile.stop, s);
+  g_profile = ProfileHook{};
+}
+
 static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, const void* omega, void* ws,
                                void* grid, hipStream_t s) {
   const Geom<float> g = make_geom<float>(d);
   const TileGeom tg = make_coil_tile_geom(d);
-  int* wsi = static_cast<int*>(ws);
-  int st = launch_zero_fill<int>(wsi, tg.plan_ints, s);
+  CacheHdr* cache = nullptr;
+  int st = plan_cache_begin(d, kPlanSpreadCoils, omega, ws, s, &cache);
+  if (st != TKBN_OK) return st;
+  int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
+  st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
   if (st != TKBN_OK) return st;
   dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
   if (g.M > 0) {
-    coil_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
+    coil_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
     TKBN_LAUNCH_CHECK("coil_bin_count_kernel");
   }
-  coil_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi);
+  coil_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
   TKBN_LAUNCH_CHECK("coil_plan_scan_kernel");
   if (g.M > 0) {
-    coil_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
+    coil_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
     TKBN_LAUNCH_CHECK("coil_bin_scatter_kernel");
   }
+  st = plan_cache_commit(d, kPlanSpreadCoils, omega, ws, s);
+  if (st != TKBN_OK) return st;
   const int64_t nrows = g.B * g.C;
   const unsigned ychunks = (unsigned)ceil_div(nrows, 64);
   zero_hot_tiles_kernel<float, 2><<<dim3((unsigned)tg.NB, ychunks, 1), dim3(256), 0, s>>>(
       g, tg, wsi, static_cast<float2*>(grid));
   TKBN_LAUNCH_CHECK("zero_hot_tiles_kernel<coils>");
+  profile_before(s);
   spread_coil_regs_kernel<<<dim3((unsigned)tg.maxwork, ychunks, 1), dim3(kRegThreads), 0, s>>>(
       g, tg, static_cast<const float2*>(kdata), wsi, static_cast<float2*>(grid));
+  profile_after(s);
   TKBN_LAUNCH_CHECK("spread_coil_regs_kernel");
   return TKBN_OK;
+}
+
+template <typename T>
+static int spread_dispatch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kdata, const void* omega,
+                           int* wsi, void* slot, void* grid, size_t lds, hipStream_t s, int ndim, int jm) {
+  TKBN_DISPATCH_DJ(ndim, jm, spread_launch, g, tg, det, kdata, omega, wsi, slot, grid, lds, s);
 }
 
 template <typename T>
@@ -1371,7 +1414,9 @@ int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const v
   // counts must start at zero: clear the whole plan area (a kernel, graph-capture safe)
   int st = launch_zero_fill<int>(wsi, nplans * tg.plan_ints, s);
   if (st != TKBN_OK) return st;
-  TKBN_DISPATCH_DJ(d->ndim, max_numpoints(d), spread_launch, g, tg, det, kdata, omega, wsi, slot, grid, lds, s);
+  st = spread_dispatch<T>(g, tg, det, kdata, omega, wsi, slot, grid, lds, s, d->ndim, max_numpoints(d));
+  if (st != TKBN_OK) return st;
+  return plan_cache_invalidate(ws, s);  // the scratch plan overwrote any cached one
 }
 
 template <typename T, int D, int JM>
@@ -1395,6 +1440,12 @@ int fwd_tiled_launch(const Geom<T>& g, const TileGeom& tg, const void* grid, con
 }
 
 template <typename T>
+static int fwd_tiled_dispatch(const Geom<T>& g, const TileGeom& tg, const void* grid, const void* omega, int* wsi,
+                              void* kdata, hipStream_t s, int ndim, int jm) {
+  TKBN_DISPATCH_DJ(ndim, jm, fwd_tiled_launch, g, tg, grid, omega, wsi, kdata, s);
+}
+
+template <typename T>
 int interp_fwd_tiled_t(const tkbn_interp_desc* d, const void* grid, const void* omega, void* kdata, void* ws,
                        hipStream_t s) {
   const Geom<T> g = make_geom<T>(d);
@@ -1403,7 +1454,9 @@ int interp_fwd_tiled_t(const tkbn_interp_desc* d, const void* grid, const void* 
   int* wsi = static_cast<int*>(ws);
   int st = launch_zero_fill<int>(wsi, nplans * tg.plan_ints, s);
   if (st != TKBN_OK) return st;
-  TKBN_DISPATCH_DJ(d->ndim, max_numpoints(d), fwd_tiled_launch, g, tg, grid, omega, wsi, kdata, s);
+  st = fwd_tiled_dispatch<T>(g, tg, grid, omega, wsi, kdata, s, d->ndim, max_numpoints(d));
+  if (st != TKBN_OK) return st;
+  return plan_cache_invalidate(ws, s);  // the scratch plan overwrote any cached one
 }
 
 // ---- multi-coil forward gather ---------------------------------------------------
@@ -1456,26 +1509,33 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
                                hipStream_t s) {
   const Geom<float> g = make_geom<float>(d);
   const TileGeom tg = make_gather_geom(d);
-  int* wsi = static_cast<int*>(ws);
-  int st = launch_zero_fill<int>(wsi, tg.plan_ints, s);
+  CacheHdr* cache = nullptr;
+  int st = plan_cache_begin(d, kPlanGatherCoils, omega, ws, s, &cache);
+  if (st != TKBN_OK) return st;
+  int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
+  st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
   if (st != TKBN_OK) return st;
   dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
-  gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
+  gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
   TKBN_LAUNCH_CHECK("gat_bin_count_kernel");
-  plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi);
+  plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
   TKBN_LAUNCH_CHECK("plan_scan_kernel<gather>");
-  gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi);
+  gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
   TKBN_LAUNCH_CHECK("gat_bin_scatter_kernel");
+  st = plan_cache_commit(d, kPlanGatherCoils, omega, ws, s);
+  if (st != TKBN_OK) return st;
   const unsigned ychunks = (unsigned)ceil_div(g.B * g.C, 64);
   const int64_t groups = ceil_div(tg.maxwork, 8) * 8;  // a multiple of 8 (xcd_work_item)
+  profile_before(s);
   gather_coils_kernel<<<dim3((unsigned)groups, ychunks, 1), dim3(kGatThreads), 0, s>>>(
       g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata));
+  profile_after(s);
   TKBN_LAUNCH_CHECK("gather_coils_kernel");
   return TKBN_OK;
 }
 
 int64_t interp_fwd_workspace_bytes(const tkbn_interp_desc* d) {
-  if (gather_coils_eligible(d)) return gather_workspace_bytes(make_gather_geom(d));
+  if (gather_coils_eligible(d)) return plan_cache_bytes(d) + gather_workspace_bytes(make_gather_geom(d));
   const TileGeom tg = make_tile_geom(d, false, true);
   const size_t lds = (size_t)tg.CC * tg.rcells * (d->dtype == TKBN_F32 ? 8 : 16);
   if (lds > (size_t)kLdsBudget) return 0;  // region too large for LDS: plain kernel
@@ -1495,6 +1555,12 @@ using namespace tkbn;
 
 extern "C" {
 
+int tkbn_profile_next_kernel(void* start_event, void* stop_event) {
+  g_profile.start = static_cast<hipEvent_t>(start_event);
+  g_profile.stop = static_cast<hipEvent_t>(stop_event);
+  return TKBN_OK;
+}
+
 int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mode) {
   if (validate_desc(desc) != TKBN_OK) return -1;
   if (mode != TKBN_ADJ_ATOMIC && mode != TKBN_ADJ_DETERMINISTIC) return -1;
@@ -1502,7 +1568,7 @@ int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mo
   const TileGeom tg = make_tile_geom(desc, det);
   int64_t n = spread_workspace_bytes(desc, tg);
   if (!det && coil_variant_eligible(desc)) {
-    const int64_t nc = coil_workspace_bytes(make_coil_tile_geom(desc));
+    const int64_t nc = plan_cache_bytes(desc) + coil_workspace_bytes(make_coil_tile_geom(desc));
     n = nc > n ? nc : n;
   }
   return n;
