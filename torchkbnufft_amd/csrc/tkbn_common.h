@@ -95,6 +95,24 @@ __device__ __forceinline__ void cmac(C& acc, C a, C b) {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Division by an invariant divisor d >= 1 as a multiply-high and shift (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994):
+// s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1, n / d = (mulhi(n, m) + n) >> s.
+// Exact for 0 <= n < 2^31 (the sum then fits in 32 bits).
+struct FastDiv {
+  uint32_t d, m, s;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  uint32_t s = 0;
+  while (s < 31 && (1ull << s) < d) ++s;
+  const uint64_t m = ((((uint64_t)1 << 32) * (((uint64_t)1 << s) - d)) / d) + 1;
+  return FastDiv{d, (uint32_t)m, s};
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) { return (__umulhi(n, f.m) + n) >> f.s; }
+__device__ __forceinline__ uint32_t fmodd(uint32_t n, const FastDiv& f) { return n - fdiv(n, f) * f.d; }
+
 // Zero-fill n elements (a kernel, not hipMemsetAsync, so callers stay safe to
 // capture in a HIP graph).
 template <typename E>

@@ -25,6 +25,8 @@ struct FftPlan1D {
   int K;                   // transform length
   int npass;
   int radix[kMaxPasses];   // radices, first pass first
+  FastDiv nb[kMaxPasses];  // K / radix: butterflies per transform in each pass
+  FastDiv ns[kMaxPasses];  // product of the radices before each pass
 };
 
 // ---------------------------------------------------------------------------
@@ -132,6 +134,27 @@ __device__ void build_twiddles(float2* tw, int K) {
   }
 }
 
+// Group-wide copy of `total` elements: store(i, load(i)) for i < total, with kCopyU
+// loads in flight per thread before their stores (a loop of single loads would
+// wait one memory latency per element).
+constexpr int kCopyU = 8;
+template <typename LoadF, typename StoreF>
+__device__ __forceinline__ void batched_copy(int total, LoadF load, StoreF store) {
+  for (int base = threadIdx.x; base < total; base += kCopyU * blockDim.x) {
+    float2 v[kCopyU];
+#pragma unroll
+    for (int u = 0; u < kCopyU; ++u) {
+      const int i = base + u * blockDim.x;
+      v[u] = i < total ? load(i) : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < kCopyU; ++u) {
+      const int i = base + u * blockDim.x;
+      if (i < total) store(i, v[u]);
+    }
+  }
+}
+
 // One Stockham radix-R pass over `ntr` transforms of length K held in LDS at
 // elem(t, k) = t * st + k * sk.  Ns = product of the radices already applied.
 // Threads enumerate butterflies b in [0, ntr * K / R): with T_MAJOR the butterfly
@@ -143,22 +166,24 @@ constexpr int kMaxBfly = 2;
 constexpr int kFftThreads = 1024;  // 16 waves per group: the stash stays small (<=128 VGPRs)
 
 template <bool INV, int R, bool T_MAJOR>
-__device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int K, int ntr, int st, int sk, int Ns) {
-  const int nb = K / R;
+__device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int K, int ntr, int st, int sk,
+                                              const FastDiv& nbd, const FastDiv& nsd, const FastDiv& ntrd) {
+  const int nb = (int)nbd.d;
+  const int Ns = (int)nsd.d;
   const int total = ntr * nb;
-  const int step = K / (Ns * R);
+  const int step = nb / Ns;  // K / (Ns * R)
   float2 vals[kMaxBfly][R];
 #pragma unroll
   for (int p = 0; p < kMaxBfly; ++p) {
     const int b = threadIdx.x + p * blockDim.x;
     if (b < total) {
-      const int t = T_MAJOR ? b / nb : b % ntr;
-      const int j = T_MAJOR ? b - t * nb : b / ntr;
+      const int t = T_MAJOR ? (int)fdiv(b, nbd) : (int)fmodd(b, ntrd);
+      const int j = T_MAJOR ? b - t * nb : (int)fdiv(b, ntrd);
       float2 x[R];
 #pragma unroll
       for (int i = 0; i < R; ++i) x[i] = buf[t * st + (j + i * nb) * sk];
       if (Ns > 1) {
-        const int k = j % Ns;
+        const int k = (int)fmodd(j, nsd);
 #pragma unroll
         for (int i = 1; i < R; ++i) {
           float2 w = tw[i * k * step];  // i*k*step < K
@@ -177,10 +202,11 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int
   for (int p = 0; p < kMaxBfly; ++p) {
     const int b = threadIdx.x + p * blockDim.x;
     if (b < total) {
-      const int t = T_MAJOR ? b / nb : b % ntr;
-      const int j = T_MAJOR ? b - t * nb : b / ntr;
-      const int k = j % Ns;
-      const int base = (j / Ns) * Ns * R + k;
+      const int t = T_MAJOR ? (int)fdiv(b, nbd) : (int)fmodd(b, ntrd);
+      const int j = T_MAJOR ? b - t * nb : (int)fdiv(b, ntrd);
+      const int jq = (int)fdiv(j, nsd);
+      const int k = j - jq * Ns;
+      const int base = jq * Ns * R + k;
 #pragma unroll
       for (int i = 0; i < R; ++i) buf[t * st + (base + i * Ns) * sk] = vals[p][i];
     }
@@ -188,19 +214,18 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int
   __syncthreads();
 }
 
+// ntrd: divider by ntr (used by the column layout, T_MAJOR = false)
 template <bool INV, bool T_MAJOR>
-__device__ void lds_fft(float2* buf, const float2* tw, const FftPlan1D& pl, int ntr, int st, int sk) {
-  int Ns = 1;
+__device__ void lds_fft(float2* buf, const float2* tw, const FftPlan1D& pl, int ntr, int st, int sk,
+                        const FastDiv& ntrd) {
   for (int p = 0; p < pl.npass; ++p) {
-    const int R = pl.radix[p];
-    switch (R) {
-      case 8: stockham_pass<INV, 8, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, Ns); break;
-      case 5: stockham_pass<INV, 5, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, Ns); break;
-      case 4: stockham_pass<INV, 4, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, Ns); break;
-      case 3: stockham_pass<INV, 3, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, Ns); break;
-      default: stockham_pass<INV, 2, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, Ns); break;
+    switch (pl.radix[p]) {
+      case 8: stockham_pass<INV, 8, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, pl.nb[p], pl.ns[p], ntrd); break;
+      case 5: stockham_pass<INV, 5, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, pl.nb[p], pl.ns[p], ntrd); break;
+      case 4: stockham_pass<INV, 4, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, pl.nb[p], pl.ns[p], ntrd); break;
+      case 3: stockham_pass<INV, 3, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, pl.nb[p], pl.ns[p], ntrd); break;
+      default: stockham_pass<INV, 2, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, pl.nb[p], pl.ns[p], ntrd); break;
     }
-    Ns *= R;
   }
 }
 
@@ -216,6 +241,7 @@ struct FftGeom {
   int out_coils;    // adjoint: 1 with smaps (combined), else C
   float scale;
   FftPlan1D plan[3];
+  FastDiv fK, fN, fQ, fC;  // last-dim grid / image extents, lines per (b, c), coils
 };
 
 // Shape of the array between passes: extents e[d] (N or K per dimension), row-major,
@@ -253,64 +279,84 @@ __global__ void __launch_bounds__(1024)
     const int64_t line0 = (int64_t)blockIdx.x * lines_per_group;
     const int64_t nlines = g.B * g.C * Q;
     const int nl = (int)((line0 + lines_per_group <= nlines) ? lines_per_group : nlines - line0);
-    for (int idx = threadIdx.x; idx < nl * K; idx += blockDim.x) {
-      const int l = idx / K, k = idx - l * K;
-      const int64_t line = line0 + l;
-      const int64_t bc = line / Q, q = line - bc * Q;
-      const int64_t b = bc / g.C, c = bc - b * g.C;
-      float2 v = make_float2(0.f, 0.f);
-      if (k < N) {
-        const int64_t n = q * N + k;
-        v = src[(b * g.image_coils + (g.image_coils == 1 ? 0 : c)) * P + n];
-        if (smaps) v = cmul(v, smaps[bc * P + n]);
-        v = cmul(v, scaling[n]);
-        v.x *= g.scale;
-        v.y *= g.scale;
-      }
-      buf[l * K + k] = v;
-    }
+    batched_copy(
+        nl * K,
+        [&](int idx) {
+          const int l = (int)fdiv(idx, g.fK), k = idx - l * K;
+          const int64_t line = line0 + l;  // < 2^31 (checked on the host)
+          const int64_t bc = fdiv((uint32_t)line, g.fQ), q = line - bc * Q;
+          const int64_t b = fdiv((uint32_t)bc, g.fC), c = bc - b * g.C;
+          float2 v = make_float2(0.f, 0.f);
+          if (k < N) {
+            const int64_t n = q * N + k;
+            v = src[(b * g.image_coils + (g.image_coils == 1 ? 0 : c)) * P + n];
+            if (smaps) v = cmul(v, smaps[bc * P + n]);
+            v = cmul(v, scaling[n]);
+            v.x *= g.scale;
+            v.y *= g.scale;
+          }
+          return v;
+        },
+        [&](int idx, float2 v) { buf[idx] = v; });
     __syncthreads();
-    lds_fft<false, true>(buf, tw, g.plan[D - 1], nl, K, 1);
-    for (int idx = threadIdx.x; idx < nl * K; idx += blockDim.x) {
-      const int l = idx / K, k = idx - l * K;
-      dst[(line0 + l) * K + k] = buf[l * K + k];
-    }
+    lds_fft<false, true>(buf, tw, g.plan[D - 1], nl, K, 1, g.fK);
+    batched_copy(
+        nl * K, [&](int idx) { return buf[idx]; }, [&](int idx, float2 v) { dst[line0 * K + idx] = v; });
   } else {
     // lines = (b, q); loop over coils in chunks of lines_per_group
     const int64_t line = blockIdx.x;
-    const int64_t b = line / Q, q = line - b * Q;
+    const int64_t b = fdiv((uint32_t)line, g.fQ), q = line - b * Q;
     const int cc = lines_per_group;
     // output owned by this thread: n = threadIdx.x (the host guarantees N <= blockDim)
     const int n = threadIdx.x;
     float2 acc = make_float2(0.f, 0.f);
     for (int64_t c0 = 0; c0 < g.C; c0 += cc) {
       const int nc = (int)((c0 + cc <= g.C) ? cc : g.C - c0);
-      for (int idx = threadIdx.x; idx < nc * K; idx += blockDim.x) {
-        const int l = idx / K, k = idx - l * K;
-        buf[l * K + k] = src[((b * g.C + c0 + l) * Q + q) * K + k];
-      }
+      batched_copy(
+          nc * K,
+          [&](int idx) {
+            const int l = (int)fdiv(idx, g.fK), k = idx - l * K;
+            return src[((b * g.C + c0 + l) * Q + q) * K + k];
+          },
+          [&](int idx, float2 v) { buf[idx] = v; });
       __syncthreads();
-      lds_fft<true, true>(buf, tw, g.plan[D - 1], nc, K, 1);
+      lds_fft<true, true>(buf, tw, g.plan[D - 1], nc, K, 1, g.fK);
       if (smaps) {
         if (n < N) {
           const float2 s = scaling[q * N + n];
-          for (int l = 0; l < nc; ++l) {
-            float2 v = cmul_conj(buf[l * K + n], s);
-            v.x *= g.scale;
-            v.y *= g.scale;
-            const float2 w = cmul_conj(v, smaps[((b * g.C + c0 + l) * Q + q) * N + n]);
-            acc.x += w.x;
-            acc.y += w.y;
+          // smaps of up to kCopyU coils in flight before they are used
+          for (int l0 = 0; l0 < nc; l0 += kCopyU) {
+            float2 sm[kCopyU];
+#pragma unroll
+            for (int u = 0; u < kCopyU; ++u)
+              sm[u] = l0 + u < nc ? smaps[((b * g.C + c0 + l0 + u) * Q + q) * N + n] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < kCopyU; ++u) {
+              if (l0 + u < nc) {
+                float2 v = cmul_conj(buf[(l0 + u) * K + n], s);
+                v.x *= g.scale;
+                v.y *= g.scale;
+                const float2 w = cmul_conj(v, sm[u]);
+                acc.x += w.x;
+                acc.y += w.y;
+              }
+            }
           }
         }
       } else {
-        for (int idx = threadIdx.x; idx < nc * N; idx += blockDim.x) {
-          const int l = idx / N, nn = idx - l * N;
-          float2 v = cmul_conj(buf[l * K + nn], scaling[q * N + nn]);
-          v.x *= g.scale;
-          v.y *= g.scale;
-          dst[((b * g.C + c0 + l) * Q + q) * N + nn] = v;
-        }
+        batched_copy(
+            nc * N,
+            [&](int idx) {
+              const int l = (int)fdiv(idx, g.fN), nn = idx - l * N;
+              float2 v = cmul_conj(buf[l * K + nn], scaling[q * N + nn]);
+              v.x *= g.scale;
+              v.y *= g.scale;
+              return v;
+            },
+            [&](int idx, float2 v) {
+              const int l = (int)fdiv(idx, g.fN), nn = idx - l * N;
+              dst[((b * g.C + c0 + l) * Q + q) * N + nn] = v;
+            });
       }
       __syncthreads();
     }
@@ -327,7 +373,7 @@ __global__ void __launch_bounds__(1024)
 template <bool INV>
 __global__ void __launch_bounds__(1024)
     fft_mid_kernel(const FftPlan1D pl, int64_t rows, int n_in, int n_out, int64_t inner, int ncol,
-                   const float2* __restrict__ src, float2* __restrict__ dst) {
+                   const FastDiv fcol, const float2* __restrict__ src, float2* __restrict__ dst) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int K = pl.K;
   float2* tw = reinterpret_cast<float2*>(smem);
@@ -339,19 +385,22 @@ __global__ void __launch_bounds__(1024)
   const int nc = (int)(col0 + ncol <= inner ? ncol : inner - col0);
   if (row >= rows) return;
   const float2* s = src + row * (int64_t)n_in * inner + col0;
-  for (int idx = threadIdx.x; idx < K * ncol; idx += blockDim.x) {
-    const int k = idx / ncol, t = idx - k * ncol;
-    float2 v = make_float2(0.f, 0.f);
-    if (k < n_in && t < nc) v = s[(int64_t)k * inner + t];
-    buf[k * ncol + t] = v;
-  }
+  batched_copy(
+      K * ncol,
+      [&](int idx) {
+        const int k = (int)fdiv(idx, fcol), t = idx - k * ncol;
+        return (k < n_in && t < nc) ? s[(int64_t)k * inner + t] : make_float2(0.f, 0.f);
+      },
+      [&](int idx, float2 v) { buf[idx] = v; });
   __syncthreads();
-  lds_fft<INV, false>(buf, tw, pl, ncol, 1, ncol);
+  lds_fft<INV, false>(buf, tw, pl, ncol, 1, ncol, fcol);
   float2* o = dst + row * (int64_t)n_out * inner + col0;
-  for (int idx = threadIdx.x; idx < n_out * ncol; idx += blockDim.x) {
-    const int k = idx / ncol, t = idx - k * ncol;
-    if (t < nc) o[(int64_t)k * inner + t] = buf[k * ncol + t];
-  }
+  batched_copy(
+      n_out * ncol, [&](int idx) { return buf[idx]; },
+      [&](int idx, float2 v) {
+        const int k = (int)fdiv(idx, fcol), t = idx - k * ncol;
+        if (t < nc) o[(int64_t)k * inner + t] = v;
+      });
 }
 
 // ---------------------------------------------------------------------------
@@ -383,6 +432,12 @@ static bool factorize(int K, FftPlan1D* pl) {
       if (pl->npass >= kMaxPasses) return false;
       pl->radix[pl->npass++] = radix[i];
     }
+  int ns = 1;
+  for (int p = 0; p < pl->npass; ++p) {
+    pl->nb[p] = make_fastdiv((uint32_t)(K / pl->radix[p]));
+    pl->ns[p] = make_fastdiv((uint32_t)ns);
+    ns *= pl->radix[p];
+  }
   return true;
 }
 
@@ -423,6 +478,20 @@ static int check_fft(int32_t ndim, const int64_t* im, const int64_t* gs, FftGeom
   return TKBN_OK;
 }
 
+// dividers of the last-dimension pass; the index arithmetic there is 32-bit
+static int set_divs(FftGeom* g) {
+  const int D = g->ndim;
+  int64_t Q = 1;
+  for (int d = 0; d < D - 1; ++d) Q *= g->N[d];
+  TKBN_REQUIRE(g->B * g->C * Q < (1LL << 31) && g->B * g->C > 0, TKBN_ERR_UNSUPPORTED,
+               "too many FFT lines (batch * coils * image lines must be < 2^31)");
+  g->fK = make_fastdiv((uint32_t)g->K[D - 1]);
+  g->fN = make_fastdiv((uint32_t)g->N[D - 1]);
+  g->fQ = make_fastdiv((uint32_t)Q);
+  g->fC = make_fastdiv((uint32_t)g->C);
+  return TKBN_OK;
+}
+
 static int mid_cols(const FftPlan1D& pl) {
   int ncol = 16;
   while (ncol > 1 && ((int64_t)(ncol + 1) * pl.K * 8 > kFftLdsBudget || ncol > max_transforms(pl))) ncol /= 2;
@@ -436,10 +505,13 @@ static int launch_mid(bool inv, const FftPlan1D& pl, int64_t rows, int n_in, int
   const int64_t groups = rows * chunks;
   TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_INVALID, "too many FFT columns");
   const size_t lds = (size_t)(pl.K + (int64_t)pl.K * ncol) * sizeof(float2);
+  const FastDiv fcol = make_fastdiv((uint32_t)ncol);
   if (inv)
-    fft_mid_kernel<true><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(pl, rows, n_in, n_out, inner, ncol, src, dst);
+    fft_mid_kernel<true><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(pl, rows, n_in, n_out, inner, ncol,
+                                                                                 fcol, src, dst);
   else
-    fft_mid_kernel<false><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(pl, rows, n_in, n_out, inner, ncol, src, dst);
+    fft_mid_kernel<false><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(pl, rows, n_in, n_out, inner, ncol,
+                                                                                  fcol, src, dst);
   TKBN_LAUNCH_CHECK("fft_mid_kernel");
   return TKBN_OK;
 }
@@ -505,6 +577,8 @@ int tkbn_fft_grid_fwd(int32_t dtype, int32_t ndim, const int64_t* im_size, const
   g.image_coils = (int)image_coils;
   g.out_coils = (int)coils;
   g.scale = (float)scale;
+  st = set_divs(&g);
+  if (st != TKBN_OK) return st;
   int64_t wa, wb;
   ws_sizes(g, &wa, &wb);
   TKBN_REQUIRE(workspace_bytes >= (wa + wb) * 8 && (wa + wb == 0 || workspace), TKBN_ERR_WORKSPACE,
@@ -563,6 +637,8 @@ int tkbn_fft_grid_adj(int32_t dtype, int32_t ndim, const int64_t* im_size, const
   g.image_coils = (int)coils;
   g.out_coils = smaps ? 1 : (int)coils;
   g.scale = (float)scale;
+  st = set_divs(&g);
+  if (st != TKBN_OK) return st;
   int64_t wa, wb;
   ws_sizes(g, &wa, &wb);
   TKBN_REQUIRE(workspace_bytes >= (wa + wb) * 8 && (wa + wb == 0 || workspace), TKBN_ERR_WORKSPACE,
