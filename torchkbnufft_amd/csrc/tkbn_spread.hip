@@ -51,6 +51,7 @@ struct TileGeom {
   int64_t o_key, o_perm, o_count, o_start, o_cursor, o_wstart, o_work, o_nwork, plan_ints;
   int64_t o_maxabs_bytes;  // byte offset of the max|kdata| slot (8 bytes)
   int64_t nkeys;           // multi-coil plan: (tile, footprint column) keys
+  int64_t o_hot, o_cold;   // multi-coil plan: scans of split / unsplit tiles
   int64_t o_rec_bytes;     // multi-coil plan: byte offset of the entry records
 };
 
@@ -516,6 +517,15 @@ constexpr int kRegBatch = 64;  // staged entries
 constexpr int kRegPitch = 65;  // float2 per staged entry: 64 rows + 1 (bank spread)
 constexpr int kRegMinC0 = -5;  // first footprint column relative to the tile, J <= 6
 constexpr int kRegKeys = kRegT1 - kRegMinC0;  // c0 values per tile: -5 .. 15
+// A tile with more entries than kRegMaxUnsplit is split into pieces of kRegHotTarget
+// entries (their groups then add into a pre-zeroed tile); the pieces are listed
+// first so the longest work starts first.  Other tiles are one work item each.
+constexpr int kRegMaxUnsplit = 1024;
+constexpr int kRegHotTarget = 384;
+
+__host__ __device__ inline int reg_splits(int v) {
+  return v <= kRegMaxUnsplit ? 1 : (v + kRegHotTarget - 1) / kRegHotTarget;
+}
 
 struct CoilRec {
   float2 w0[6];  // conj(phase * w0[j]) along dimension 0
@@ -663,20 +673,50 @@ __global__ void __launch_bounds__(1024) coil_plan_scan_kernel(const TileGeom tg,
   }
   if (threadIdx.x == 0) start[tg.nkeys] = cursor[NB];
   __syncthreads();
-  // 3. cursors, splits and the work list
+  // 3. cursors, splits and the work list: the pieces of split tiles first
   for (int64_t i = threadIdx.x; i < tg.nkeys; i += blockDim.x) cursor[i] = start[i];
+  int* hot = plan + tg.o_hot;    // NB + 1: pieces of split tiles, then their scan
+  int* cold = plan + tg.o_cold;  // NB + 1: 1 for unsplit tiles, then their scan
   for (int t = threadIdx.x; t < NB; t += blockDim.x) {
     const int v = start[(int64_t)(t + 1) * kRegKeys] - start[(int64_t)t * kRegKeys];
-    work[t] = v <= tg.target ? 1 : (v + tg.target - 1) / tg.target;
+    const int ns = reg_splits(v);
+    work[t] = ns > 1 ? ns : 0;
+    wstart[t] = ns > 1 ? 0 : 1;
   }
   __syncthreads();
-  block_exclusive_scan(work, wstart, NB, sbuf);
+  block_exclusive_scan(work, hot, NB, sbuf);
+  block_exclusive_scan(wstart, cold, NB, sbuf);
+  const int nhot = hot[NB];
   const int cap = (int)tg.maxwork;
   for (int t = threadIdx.x; t < NB; t += blockDim.x) {
-    const int a = wstart[t], e = min(wstart[t + 1], cap);
+    const int ns = hot[t + 1] - hot[t];
+    wstart[t] = ns > 0 ? hot[t] : nhot + cold[t];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int ns = max(hot[t + 1] - hot[t], 1);
+    const int a = wstart[t], e = min(a + ns, cap);
     for (int w = a; w < e; ++w) work[w] = t;
   }
-  if (threadIdx.x == 0) plan[tg.o_nwork] = min(wstart[NB], cap);
+  if (threadIdx.x == 0) plan[tg.o_nwork] = min(nhot + cold[NB], cap);
+}
+
+// zeroes the tiles whose pieces add into the grid
+__global__ void __launch_bounds__(256)
+    coil_zero_hot_kernel(const Geom<float> g, const TileGeom tg, const int* __restrict__ ws, float2* __restrict__ grid) {
+  const int t = blockIdx.x;
+  const int* start = ws + tg.o_start;
+  const int v = start[(int64_t)(t + 1) * kRegKeys] - start[(int64_t)t * kRegKeys];
+  if (reg_splits(v) <= 1) return;
+  const int64_t nrows = g.B * g.C;
+  const int64_t r0 = (int64_t)blockIdx.y * 64;
+  const int64_t r1 = r0 + 64 < nrows ? r0 + 64 : nrows;
+  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kRegT0;
+  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kRegT1;
+  const int cell = threadIdx.x;  // 256 = kRegT0 * kRegT1
+  const int k0 = o0 + (cell >> 4), k1 = o1 + (cell & 15);
+  if (k0 >= g.K[0] || k1 >= g.K[1]) return;
+  for (int64_t r = r0; r < r1; ++r) grid[r * g.G + (int64_t)k0 * g.stride[0] + k1] = make_float2(0.f, 0.f);
 }
 
 // acc[C0 + j] += u * w1[j] for the footprint columns that fall inside the tile
@@ -728,7 +768,6 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
   const int* wstart = plan + tg.o_wstart;
   const int t = plan[tg.o_work + wi];
   const int split = (int)(wi - wstart[t]);
-  const int nsplit = wstart[t + 1] - wstart[t];
   const int64_t nrows = g.B * g.C;
   const int64_t rbase = (int64_t)blockIdx.y * 64;
   const int ncc = (int)((rbase + 64 < nrows ? rbase + 64 : nrows) - rbase);
@@ -736,8 +775,9 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
   const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kRegT1;
   const int cstart = plan[tg.o_start + t * kRegKeys];
   const int V = plan[tg.o_start + (t + 1) * kRegKeys] - cstart;
-  const int i0 = split * tg.target;
-  const int i1 = min(i0 + tg.target, V);
+  const int nsplit = reg_splits(V);
+  const int i0 = nsplit > 1 ? split * kRegHotTarget : 0;
+  const int i1 = nsplit > 1 ? min(i0 + kRegHotTarget, V) : V;
   const int* perm = plan + tg.o_perm + cstart;
   const CoilRec* recs = reinterpret_cast<const CoilRec*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) + cstart;
   const int J0 = g.J[0];
@@ -809,28 +849,35 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
     __syncthreads();
   }
 
-  // every owned cell inside the grid is written (split tiles were zeroed first)
-  if (lane < ncc) {
-    float2* dst_row = grid + (rbase + lane) * g.G;
+  // Flush: every owned cell inside the grid is written (split tiles were zeroed first
+  // and add).  The rows go out through LDS in four rounds of four tile rows so that
+  // each store instruction writes 16 consecutive cells (128 B) of 4 coil rows.
+  for (int round = 0; round < 4; ++round) {
+    if (wave >> 1 == round) {
+      const int slot0 = (wave & 1) * kRegRowsPerWave;  // rows 4*round + slot
 #pragma unroll
-    for (int h = 0; h < kRegRowsPerWave; ++h) {
-      const int k0 = o0 + myrow + h;
-      if (k0 < g.K[0]) {
+      for (int h = 0; h < kRegRowsPerWave; ++h)
 #pragma unroll
-        for (int c = 0; c < kRegT1; ++c) {
-          const int k1 = o1 + c;
-          if (k1 < g.K[1]) {
-            float2* dst = dst_row + (int64_t)k0 * g.stride[0] + k1;
-            if (nsplit == 1) {
-              *dst = acc[h][c];
-            } else {
-              unsafeAtomicAdd(&dst->x, acc[h][c].x);
-              unsafeAtomicAdd(&dst->y, acc[h][c].y);
-            }
-          }
+        for (int c = 0; c < kRegT1; ++c) vals[((slot0 + h) * kRegT1 + c) * kRegPitch + lane] = acc[h][c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < (4 * kRegT1 * 64) / kRegThreads; ++q) {
+      const int e = threadIdx.x + q * kRegThreads;
+      const int c = e & (kRegT1 - 1), coil = (e >> 4) & 63, slot = e >> 10;
+      const int k0 = o0 + 4 * round + slot, k1 = o1 + c;
+      if (coil < ncc && k0 < g.K[0] && k1 < g.K[1]) {
+        const float2 v = vals[((slot * kRegT1) + c) * kRegPitch + coil];
+        float2* dst = grid + (rbase + coil) * g.G + (int64_t)k0 * g.stride[0] + k1;
+        if (nsplit == 1) {
+          *dst = v;
+        } else {
+          unsafeAtomicAdd(&dst->x, v.x);
+          unsafeAtomicAdd(&dst->y, v.y);
         }
       }
     }
+    __syncthreads();
   }
 }
 
@@ -1309,7 +1356,7 @@ static TileGeom make_coil_tile_geom(const tkbn_interp_desc* d) {
   }
   const int64_t M = d->klength;
   tg.maxent = per_sample * M;
-  tg.maxwork = tg.NB + tg.maxent / tg.target + 1;
+  tg.maxwork = tg.NB + tg.maxent / kRegHotTarget + 1;
   tg.nkeys = (int64_t)tg.NB * kRegKeys;
   int64_t o = 0;
   tg.o_key = o;
@@ -1320,6 +1367,8 @@ static TileGeom make_coil_tile_geom(const tkbn_interp_desc* d) {
   tg.o_wstart = o; o += tg.NB + 1;
   tg.o_work = o; o += (tg.maxwork > tg.NB ? tg.maxwork : tg.NB);
   tg.o_nwork = o; o += 1;
+  tg.o_hot = o; o += tg.NB + 1;
+  tg.o_cold = o; o += tg.NB + 1;
   o = (o + 3) & ~3LL;  // 16-byte aligned records
   tg.plan_ints = o;
   tg.o_rec_bytes = o * 4;
@@ -1383,9 +1432,9 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
   if (st != TKBN_OK) return st;
   const int64_t nrows = g.B * g.C;
   const unsigned ychunks = (unsigned)ceil_div(nrows, 64);
-  zero_hot_tiles_kernel<float, 2><<<dim3((unsigned)tg.NB, ychunks, 1), dim3(256), 0, s>>>(
+  coil_zero_hot_kernel<<<dim3((unsigned)tg.NB, ychunks, 1), dim3(kRegT0 * kRegT1), 0, s>>>(
       g, tg, wsi, static_cast<float2*>(grid));
-  TKBN_LAUNCH_CHECK("zero_hot_tiles_kernel<coils>");
+  TKBN_LAUNCH_CHECK("coil_zero_hot_kernel");
   profile_before(s);
   spread_coil_regs_kernel<<<dim3((unsigned)tg.maxwork, ychunks, 1), dim3(kRegThreads), 0, s>>>(
       g, tg, static_cast<const float2*>(kdata), wsi, static_cast<float2*>(grid));
