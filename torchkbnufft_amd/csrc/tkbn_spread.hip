@@ -962,12 +962,58 @@ __global__ void __launch_bounds__(256)
   reinterpret_cast<GatRec*>(reinterpret_cast<char*>(ws) + tg.o_rec_bytes)[pos] = rec;
 }
 
-// Work item of group b in a grid of n groups, n a multiple of 8: consecutive groups go
-// to different XCDs (round robin), so item = (b mod 8) * (n / 8) + b / 8 gives each
-// XCD a contiguous run of tiles and neighbouring tiles share their halo through that
-// XCD's L2.  The map is a bijection of [0, n) because n is a multiple of 8.
-__device__ __forceinline__ int64_t xcd_work_item(int64_t b, int64_t n) {
-  return (b % 8) * (n / 8) + b / 8;
+// Scan of the gather plan: tiles with more than `target` entries are cut into
+// pieces of `target`, listed first (longest work first); the rest follow, one each.
+__global__ void __launch_bounds__(1024) gat_plan_scan_kernel(const TileGeom tg, int* __restrict__ ws,
+                                                            const CacheHdr* cache) {
+  if (cache->ok) return;  // cached plan reused
+  __shared__ int sbuf[1024];
+  int* plan = ws;
+  const int* count = plan + tg.o_count;
+  int* start = plan + tg.o_start;
+  int* cursor = plan + tg.o_cursor;
+  int* wstart = plan + tg.o_wstart;
+  int* work = plan + tg.o_work;
+  int* hot = plan + tg.o_hot;
+  int* cold = plan + tg.o_cold;
+  const int NB = tg.NB;
+  block_exclusive_scan(count, start, NB, sbuf);
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    cursor[t] = start[t];
+    const int v = count[t];
+    const int ns = v <= tg.target ? 1 : (v + tg.target - 1) / tg.target;
+    work[t] = ns > 1 ? ns : 0;
+    wstart[t] = (ns == 1 && v > 0) ? 1 : 0;  // empty tiles gather nothing: no item
+  }
+  __syncthreads();
+  block_exclusive_scan(work, hot, NB, sbuf);
+  block_exclusive_scan(wstart, cold, NB, sbuf);
+  const int nhot = hot[NB];
+  const int cap = (int)tg.maxwork;
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int ns = hot[t + 1] - hot[t];
+    wstart[t] = ns > 0 ? hot[t] : nhot + cold[t];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int ns = hot[t + 1] - hot[t];
+    const int items = ns > 0 ? ns : cold[t + 1] - cold[t];
+    const int a = wstart[t], e = min(a + items, cap);
+    for (int w = a; w < e; ++w) work[w] = t;
+  }
+  if (threadIdx.x == 0) plan[tg.o_nwork] = min(nhot + cold[NB], cap);
+}
+
+// Work item of group b.  Consecutive groups go to different XCDs (round robin), so
+// group b runs on XCD x = b mod 8 as that XCD's k-th group, k = b / 8.  Items are
+// dealt to the XCDs in chunks of kXcdChunk neighbouring tiles (chunk c -> XCD c mod 8):
+// tiles of a chunk share halo cells through one L2, and the hot tiles at the head of
+// the list spread over all XCDs.  A bijection of [0, n) for n a multiple of
+// 8 * kXcdChunk.
+constexpr int kXcdChunk = 4;
+__device__ __forceinline__ int64_t xcd_work_item(int64_t b) {
+  const int64_t x = b % 8, k = b / 8;
+  return kXcdChunk * (x + 8 * (k / kXcdChunk)) + k % kXcdChunk;
 }
 
 __global__ void __launch_bounds__(kGatThreads)
@@ -977,7 +1023,7 @@ __global__ void __launch_bounds__(kGatThreads)
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int* plan = ws;
-  const int64_t wi = xcd_work_item(blockIdx.x, gridDim.x);
+  const int64_t wi = xcd_work_item(blockIdx.x);
   if (wi >= plan[tg.o_nwork]) return;
   const int* wstart = plan + tg.o_wstart;
   const int t = plan[tg.o_work + wi];
@@ -1544,6 +1590,8 @@ static TileGeom make_gather_geom(const tkbn_interp_desc* d) {
   tg.o_wstart = o; o += tg.NB + 1;
   tg.o_work = o; o += (tg.maxwork > tg.NB ? tg.maxwork : tg.NB);
   tg.o_nwork = o; o += 1;
+  tg.o_hot = o; o += tg.NB + 1;
+  tg.o_cold = o; o += tg.NB + 1;
   o = (o + 3) & ~3LL;
   tg.plan_ints = o;
   tg.o_rec_bytes = o * 4;
@@ -1567,14 +1615,14 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
   dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
   gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
   TKBN_LAUNCH_CHECK("gat_bin_count_kernel");
-  plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
-  TKBN_LAUNCH_CHECK("plan_scan_kernel<gather>");
+  gat_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
+  TKBN_LAUNCH_CHECK("gat_plan_scan_kernel");
   gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
   TKBN_LAUNCH_CHECK("gat_bin_scatter_kernel");
   st = plan_cache_commit(d, kPlanGatherCoils, omega, ws, s);
   if (st != TKBN_OK) return st;
   const unsigned ychunks = (unsigned)ceil_div(g.B * g.C, 64);
-  const int64_t groups = ceil_div(tg.maxwork, 8) * 8;  // a multiple of 8 (xcd_work_item)
+  const int64_t groups = ceil_div(tg.maxwork, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
   profile_before(s);
   gather_coils_kernel<<<dim3((unsigned)groups, ychunks, 1), dim3(kGatThreads), 0, s>>>(
       g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata));
