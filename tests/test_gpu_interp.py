@@ -312,3 +312,32 @@ def test_plan_cache_reuse_and_invalidation():
     ref4 = tp.kb_table_interp(plan4, x[:, :2].cpu(), om.cpu())
     assert rel_l2(fw4(x, om)[:, :2].cpu(), ref4) < C64_TOL
     assert torch.equal(fw(x, om), f2)
+
+
+@pytest.mark.parametrize("rows,deterministic", [(2, False), (2, True), (40, False)])
+def test_adjoint_nonfinite_kdata_propagates(rows, deterministic):
+    """A NaN sample lands as NaN exactly on its footprint cells (fixed-point paths hand
+    non-finite inputs to the float kernel)."""
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    print(f"[case] adjoint NaN propagation rows={rows} det={deterministic}", flush=True)
+    gen = torch.Generator().manual_seed(43)
+    grid, M = (64, 48), 5000
+    om = create_ktraj(2, M, gen, torch.float32)
+    y = gi.cplx((1, rows, M), gen)
+    y[0, :, 17] = complex(float("nan"), 0.0)
+    ad = tkbn.KbInterpAdjoint(im_size=(32, 24), grid_size=grid, dtype=torch.complex64).to(DEV)
+    args = (om.to(DEV), ad._tables(), ad.n_shift, ad.numpoints, ad.table_oversamp, ad.offsets, grid)
+    out = _interp.table_interp_adjoint(y.to(DEV), *args, deterministic=deterministic).cpu()
+    nan = torch.isnan(out.real) | torch.isnan(out.imag)
+    unit = torch.zeros(1, 1, M, dtype=torch.complex64)
+    unit[0, 0, 17] = 1.0
+    plan = tp.Plan((32, 24), grid, dtype=torch.complex64)
+    foot = tp.kb_table_interp_adjoint(plan, unit, om) != 0
+    assert torch.equal(nan[:, :1], foot) and bool(nan.all(dim=1).eq(nan.any(dim=1)).all())
+    # every other cell matches the finite computation without that sample
+    y2 = y.clone()
+    y2[0, :, 17] = 0
+    ref = _interp.table_interp_adjoint(y2.to(DEV), *args, deterministic=deterministic).cpu()
+    keep = ~nan
+    assert rel_l2(out[keep], ref[keep]) < 1e-6

@@ -174,7 +174,9 @@ __device__ __forceinline__ void for_each_tile(const TileGeom& tg, int tl[3][3], 
 
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
-    bin_count_kernel(const Geom<T> g, const TileGeom tg, const T* __restrict__ omega, int* __restrict__ ws) {
+    bin_count_kernel(const Geom<T> g, const TileGeom tg, const T* __restrict__ omega, int* __restrict__ ws,
+                     const CacheHdr* cache) {
+  if (cache->ok) return;  // cached plan reused
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   const int64_t b = blockIdx.z;
@@ -187,7 +189,9 @@ __global__ void __launch_bounds__(256)
 
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
-    bin_scatter_kernel(const Geom<T> g, const TileGeom tg, const T* __restrict__ omega, int* __restrict__ ws) {
+    bin_scatter_kernel(const Geom<T> g, const TileGeom tg, const T* __restrict__ omega, int* __restrict__ ws,
+                       const CacheHdr* cache) {
+  if (cache->ok) return;  // cached plan reused
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   const int64_t b = blockIdx.z;
@@ -273,14 +277,19 @@ struct uint_of<double> {
   using type = unsigned long long;
 };
 
+// max |re|, |im| over kdata, or +inf when kdata or omega holds a non-finite value
+// (fixed-point accumulation cannot carry those; the float-atomic kernel then runs)
 template <typename T>
-__global__ void __launch_bounds__(256) maxabs_kernel(const T* __restrict__ x, int64_t n, void* slot) {
+__global__ void __launch_bounds__(256) maxabs_kernel(const T* __restrict__ x, int64_t n, const T* __restrict__ om,
+                                                     int64_t nom, void* slot) {
   using U = typename uint_of<T>::type;
   T mx = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const T v = fabs(x[i]);
-    mx = v > mx ? v : mx;  // NaN never wins
+    mx = isfinite(v) ? (v > mx ? v : mx) : (T)INFINITY;
   }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nom; i += (int64_t)gridDim.x * blockDim.x)
+    if (!isfinite(om[i])) mx = (T)INFINITY;
   // wave reduce then one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
     const T o = __shfl_down(mx, off, 64);
@@ -342,13 +351,22 @@ struct AccOf<T, true> {
   using type = unsigned long long;
 };
 
-template <typename T, bool DET, int D, int JM>
+// FIXED: 64-bit fixed-point LDS accumulation (LDS integer atomics are much faster
+// than float ones; the sum is also order independent).  It exits when the max|kdata|
+// slot says a non-finite value is present; the float kernel launched with
+// `fallback` = 1 runs only in that case, so NaN and inf still propagate.
+template <typename T, bool FIXED, int D, int JM>
 __global__ void __launch_bounds__(256)
     spread_tiles_kernel(const Geom<T> g, const TileGeom tg, const cplx<T>* __restrict__ kdata,
                         const T* __restrict__ omega, const int* __restrict__ ws, const void* __restrict__ maxabs_slot,
-                        cplx<T>* __restrict__ grid) {
+                        cplx<T>* __restrict__ grid, const int fallback) {
   using C2 = cplx<T>;
-  using A = typename AccOf<T, DET>::type;
+  using A = typename AccOf<T, FIXED>::type;
+  T amax = 0;
+  if (FIXED || fallback) {
+    memcpy(&amax, maxabs_slot, sizeof(T));
+    if (FIXED != (bool)isfinite(amax)) return;  // the other kernel of the pair handles this call
+  }
   // 16-byte aligned: 8-byte LDS atomics (double / int64 accumulators) need natural
   // alignment, and the static LDS above this array is not a multiple of 8 bytes
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -382,9 +400,7 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
 
   double scale = 1.0;
-  if constexpr (DET) {
-    T amax;
-    memcpy(&amax, maxabs_slot, sizeof(T));
+  if constexpr (FIXED) {
     const double bound = 2.0 * (double)amax * (double)(V > 0 ? V : 1);
     int e = 50;
     if (bound > 0 && bound < 1e300) e = 60 - (ilogb(bound) + 1);
@@ -415,7 +431,7 @@ __global__ void __launch_bounds__(256)
       A* are = acc + (int64_t)(2 * c) * tg.tcells;
       A* aim = are + tg.tcells;
       auto add = [&](int cell, C2 x) {
-        if constexpr (DET) {
+        if constexpr (FIXED) {
           atomicAdd(are + cell, (unsigned long long)llrint((double)x.x * scale));
           atomicAdd(aim + cell, (unsigned long long)llrint((double)x.y * scale));
         } else {
@@ -475,7 +491,7 @@ __global__ void __launch_bounds__(256)
     }
     if (!in) continue;
     C2 val;
-    if constexpr (DET) {
+    if constexpr (FIXED) {
       val.x = (T)((double)(long long)acc[(2 * c) * tg.tcells + cell] * inv);
       val.y = (T)((double)(long long)acc[(2 * c + 1) * tg.tcells + cell] * inv);
     } else {
@@ -1285,7 +1301,7 @@ static TileGeom make_tile_geom(const tkbn_interp_desc* d, bool det, bool fwd = f
   }
   rcells = rs;
   tg.rcells = rcells;
-  const int accb = det ? 8 : (d->dtype == TKBN_F32 ? 4 : 8);
+  const int accb = 8;  // 64-bit fixed point (float data, deterministic) or double
   const int64_t nrows = d->omega_batched ? d->coils : d->batch * d->coils;
   int cc = fwd ? (int)(kLdsBudget / ((int64_t)rcells * (d->dtype == TKBN_F32 ? 8 : 16)))
                : (int)(kLdsBudget / (2LL * tcells * accb));
@@ -1328,48 +1344,54 @@ static int64_t spread_workspace_bytes(const tkbn_interp_desc* d, const TileGeom&
 }
 
 static size_t spread_lds_bytes(const tkbn_interp_desc* d, const TileGeom& tg, bool det) {
-  const int accb = det ? 8 : (d->dtype == TKBN_F32 ? 4 : 8);
-  return (size_t)tg.CC * 2 * tg.tcells * accb;
+  (void)d;
+  (void)det;
+  return (size_t)tg.CC * 2 * tg.tcells * 8;  // 8-byte accumulators (make_tile_geom)
 }
 
 template <typename T, int D, int JM>
 int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kdata, const void* omega, int* ws,
-                  void* maxabs_slot, void* grid, size_t lds, hipStream_t s) {
+                  void* maxabs_slot, void* grid, size_t lds, hipStream_t s, const CacheHdr* cache) {
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t zdim = g.batched ? g.B : 1;
   if (g.M > 0) {
     dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
-    bin_count_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws);
+    bin_count_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
     TKBN_LAUNCH_CHECK("bin_count_kernel");
   }
-  plan_scan_kernel<<<dim3((unsigned)zdim), dim3(1024), 0, s>>>(tg, ws);
+  plan_scan_kernel<<<dim3((unsigned)zdim), dim3(1024), 0, s>>>(tg, ws, cache);
   TKBN_LAUNCH_CHECK("plan_scan_kernel");
   if (g.M > 0) {
     dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
-    bin_scatter_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws);
+    bin_scatter_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
     TKBN_LAUNCH_CHECK("bin_scatter_kernel");
   }
   const unsigned ychunks = (unsigned)ceil_div(nrows, tg.CC);
-  if (det) {
-    int st = launch_zero_fill<unsigned long long>(static_cast<unsigned long long*>(maxabs_slot), 1, s);
-    if (st != TKBN_OK) return st;
-    const int64_t n = g.B * g.C * g.M * 2;
-    if (n > 0) {
-      const int64_t blocks = ceil_div(n, 256) < 2048 ? ceil_div(n, 256) : 2048;
-      maxabs_kernel<T><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(static_cast<const T*>(kdata), n, maxabs_slot);
-      TKBN_LAUNCH_CHECK("maxabs_kernel");
-    }
-    spread_tiles_kernel<T, true, D, JM><<<dim3((unsigned)tg.maxwork, ychunks, (unsigned)zdim), dim3(256), lds, s>>>(
-        g, tg, static_cast<const cplx<T>*>(kdata), static_cast<const T*>(omega), ws, maxabs_slot,
-        static_cast<cplx<T>*>(grid));
-    TKBN_LAUNCH_CHECK("spread_tiles_kernel<det>");
-  } else {
+  const dim3 grid3((unsigned)tg.maxwork, ychunks, (unsigned)zdim);
+  if (!det) {
     zero_hot_tiles_kernel<T, D><<<dim3((unsigned)tg.NB, ychunks, (unsigned)zdim), dim3(256), 0, s>>>(
         g, tg, ws, static_cast<cplx<T>*>(grid));
     TKBN_LAUNCH_CHECK("zero_hot_tiles_kernel");
-    spread_tiles_kernel<T, false, D, JM><<<dim3((unsigned)tg.maxwork, ychunks, (unsigned)zdim), dim3(256), lds, s>>>(
-        g, tg, static_cast<const cplx<T>*>(kdata), static_cast<const T*>(omega), ws, maxabs_slot,
-        static_cast<cplx<T>*>(grid));
+  }
+  const auto* kd = static_cast<const cplx<T>*>(kdata);
+  const auto* om = static_cast<const T*>(omega);
+  auto* gr = static_cast<cplx<T>*>(grid);
+  if (det || std::is_same<T, float>::value) {
+    // fixed point; the float kernel only runs if kdata or omega is not finite
+    int st = launch_zero_fill<unsigned long long>(static_cast<unsigned long long*>(maxabs_slot), 1, s);
+    if (st != TKBN_OK) return st;
+    const int64_t n = g.B * g.C * g.M * 2;
+    const int64_t nom = (g.batched ? g.B : 1) * D * g.M;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(std::max(n, nom), 256), 2048));
+    maxabs_kernel<T><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(static_cast<const T*>(kdata), n, om, nom,
+                                                                   maxabs_slot);
+    TKBN_LAUNCH_CHECK("maxabs_kernel");
+    spread_tiles_kernel<T, true, D, JM><<<grid3, dim3(256), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 0);
+    TKBN_LAUNCH_CHECK("spread_tiles_kernel<fixed>");
+    spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(256), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 1);
+    TKBN_LAUNCH_CHECK("spread_tiles_kernel<non-finite>");
+  } else {
+    spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(256), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 0);
     TKBN_LAUNCH_CHECK("spread_tiles_kernel");
   }
   return TKBN_OK;
@@ -1489,10 +1511,14 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
   return TKBN_OK;
 }
 
+constexpr int kPlanSpreadTiles = 3;  // + 1 for the deterministic tile geometry
+constexpr int kPlanFwdTiles = 5;
+
 template <typename T>
 static int spread_dispatch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kdata, const void* omega,
-                           int* wsi, void* slot, void* grid, size_t lds, hipStream_t s, int ndim, int jm) {
-  TKBN_DISPATCH_DJ(ndim, jm, spread_launch, g, tg, det, kdata, omega, wsi, slot, grid, lds, s);
+                           int* wsi, void* slot, void* grid, size_t lds, hipStream_t s, const CacheHdr* cache,
+                           int ndim, int jm) {
+  TKBN_DISPATCH_DJ(ndim, jm, spread_launch, g, tg, det, kdata, omega, wsi, slot, grid, lds, s, cache);
 }
 
 template <typename T>
@@ -1504,27 +1530,31 @@ int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const v
   const TileGeom tg = make_tile_geom(d, det);
   const size_t lds = spread_lds_bytes(d, tg, det);
   const int64_t nplans = d->omega_batched ? d->batch : 1;
-  int* wsi = static_cast<int*>(ws);
-  void* slot = static_cast<char*>(ws) + nplans * tg.plan_ints * 4;
-  // counts must start at zero: clear the whole plan area (a kernel, graph-capture safe)
-  int st = launch_zero_fill<int>(wsi, nplans * tg.plan_ints, s);
+  const int kind = kPlanSpreadTiles + (det ? 1 : 0);
+  CacheHdr* cache = nullptr;
+  int st = plan_cache_begin(d, kind, omega, ws, s, &cache);
   if (st != TKBN_OK) return st;
-  st = spread_dispatch<T>(g, tg, det, kdata, omega, wsi, slot, grid, lds, s, d->ndim, max_numpoints(d));
+  int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
+  void* slot = reinterpret_cast<char*>(wsi) + nplans * tg.plan_ints * 4;
+  // counts must start at zero: clear the plan area unless the cached plan is reused
+  st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
   if (st != TKBN_OK) return st;
-  return plan_cache_invalidate(ws, s);  // the scratch plan overwrote any cached one
+  st = spread_dispatch<T>(g, tg, det, kdata, omega, wsi, slot, grid, lds, s, cache, d->ndim, max_numpoints(d));
+  if (st != TKBN_OK) return st;
+  return plan_cache_commit(d, kind, omega, ws, s);
 }
 
 template <typename T, int D, int JM>
 int fwd_tiled_launch(const Geom<T>& g, const TileGeom& tg, const void* grid, const void* omega, int* ws, void* kdata,
-                     hipStream_t s) {
+                     hipStream_t s, const CacheHdr* cache) {
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t zdim = g.batched ? g.B : 1;
   dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
-  bin_count_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws);
+  bin_count_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
   TKBN_LAUNCH_CHECK("bin_count_kernel<own>");
-  plan_scan_kernel<<<dim3((unsigned)zdim), dim3(1024), 0, s>>>(tg, ws);
+  plan_scan_kernel<<<dim3((unsigned)zdim), dim3(1024), 0, s>>>(tg, ws, cache);
   TKBN_LAUNCH_CHECK("plan_scan_kernel<own>");
-  bin_scatter_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws);
+  bin_scatter_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
   TKBN_LAUNCH_CHECK("bin_scatter_kernel<own>");
   const size_t lds = (size_t)tg.CC * tg.rcells * sizeof(cplx<T>);
   interp_fwd_tiled_kernel<T, D, JM>
@@ -1536,8 +1566,8 @@ int fwd_tiled_launch(const Geom<T>& g, const TileGeom& tg, const void* grid, con
 
 template <typename T>
 static int fwd_tiled_dispatch(const Geom<T>& g, const TileGeom& tg, const void* grid, const void* omega, int* wsi,
-                              void* kdata, hipStream_t s, int ndim, int jm) {
-  TKBN_DISPATCH_DJ(ndim, jm, fwd_tiled_launch, g, tg, grid, omega, wsi, kdata, s);
+                              void* kdata, hipStream_t s, const CacheHdr* cache, int ndim, int jm) {
+  TKBN_DISPATCH_DJ(ndim, jm, fwd_tiled_launch, g, tg, grid, omega, wsi, kdata, s, cache);
 }
 
 template <typename T>
@@ -1546,12 +1576,15 @@ int interp_fwd_tiled_t(const tkbn_interp_desc* d, const void* grid, const void* 
   const Geom<T> g = make_geom<T>(d);
   const TileGeom tg = make_tile_geom(d, false, true);
   const int64_t nplans = d->omega_batched ? d->batch : 1;
-  int* wsi = static_cast<int*>(ws);
-  int st = launch_zero_fill<int>(wsi, nplans * tg.plan_ints, s);
+  CacheHdr* cache = nullptr;
+  int st = plan_cache_begin(d, kPlanFwdTiles, omega, ws, s, &cache);
   if (st != TKBN_OK) return st;
-  st = fwd_tiled_dispatch<T>(g, tg, grid, omega, wsi, kdata, s, d->ndim, max_numpoints(d));
+  int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
+  st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
   if (st != TKBN_OK) return st;
-  return plan_cache_invalidate(ws, s);  // the scratch plan overwrote any cached one
+  st = fwd_tiled_dispatch<T>(g, tg, grid, omega, wsi, kdata, s, cache, d->ndim, max_numpoints(d));
+  if (st != TKBN_OK) return st;
+  return plan_cache_commit(d, kPlanFwdTiles, omega, ws, s);
 }
 
 // ---- multi-coil forward gather ---------------------------------------------------
@@ -1636,7 +1669,7 @@ int64_t interp_fwd_workspace_bytes(const tkbn_interp_desc* d) {
   const TileGeom tg = make_tile_geom(d, false, true);
   const size_t lds = (size_t)tg.CC * tg.rcells * (d->dtype == TKBN_F32 ? 8 : 16);
   if (lds > (size_t)kLdsBudget) return 0;  // region too large for LDS: plain kernel
-  return spread_workspace_bytes(d, tg);
+  return plan_cache_bytes(d) + spread_workspace_bytes(d, tg);
 }
 
 int interp_fwd_tiled(const tkbn_interp_desc* d, const void* grid, const void* omega, void* kdata, void* ws,
@@ -1663,7 +1696,7 @@ int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mo
   if (mode != TKBN_ADJ_ATOMIC && mode != TKBN_ADJ_DETERMINISTIC) return -1;
   const bool det = mode == TKBN_ADJ_DETERMINISTIC;
   const TileGeom tg = make_tile_geom(desc, det);
-  int64_t n = spread_workspace_bytes(desc, tg);
+  int64_t n = plan_cache_bytes(desc) + spread_workspace_bytes(desc, tg);
   if (!det && coil_variant_eligible(desc)) {
     const int64_t nc = plan_cache_bytes(desc) + coil_workspace_bytes(make_coil_tile_geom(desc));
     n = nc > n ? nc : n;
