@@ -96,8 +96,16 @@ typedef struct tkbn_interp_desc {
   int64_t coils;
   int64_t klength;
   int32_t omega_batched;
-  int32_t reserved;
+  int32_t flags; /* TKBN_PLAN_REUSE or 0 */
 } tkbn_interp_desc;
+
+/*
+ * desc->flags: TKBN_PLAN_REUSE asserts that the workspace already holds the plan that
+ * an earlier call with this workspace built for bitwise the same omega, tables and
+ * geometry; the call then skips building and checking it (no plan kernels at all,
+ * for HIP-graph replays of fixed trajectories).  Passing it otherwise is undefined.
+ */
+#define TKBN_PLAN_REUSE 1
 
 TKBN_API int tkbn_abi_version(void);
 TKBN_API const char* tkbn_last_error(void);

@@ -9,6 +9,7 @@ by hand-written HIP kernels (``libtkbn_hip.so``, loaded with ctypes) around
 import torch  # noqa: F401  (load torch's HIP runtime before libtkbn_hip.so)
 
 from . import functional, modules
+from .batched import BatchedKbNufft
 from ._math import absolute, complex_mult, complex_sign, conj_complex_mult, imag_exp, inner_product
 from ._nufft import utils as nufft_utils
 from ._nufft.interp import clear_plan_cache, deterministic_enabled, set_deterministic
@@ -33,4 +34,5 @@ __all__ = [
     "nufft_utils",
     "set_deterministic",
     "clear_plan_cache",
+    "BatchedKbNufft",
 ]

@@ -27,6 +27,8 @@ TKBN_F64 = 1
 TKBN_ADJ_ATOMIC = 0
 TKBN_ADJ_DETERMINISTIC = 1
 
+TKBN_PLAN_REUSE = 1
+
 _c_i32 = ctypes.c_int32
 _c_i64 = ctypes.c_int64
 _c_dbl = ctypes.c_double
@@ -48,7 +50,7 @@ class InterpDesc(ctypes.Structure):
         ("coils", _c_i64),
         ("klength", _c_i64),
         ("omega_batched", _c_i32),
-        ("reserved", _c_i32),
+        ("flags", _c_i32),
     ]
 
 

@@ -87,7 +87,7 @@ def crop_deapod(grid: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_
         if tuple(smaps.shape) != (B, C) + im:
             raise ValueError(f"smaps shape {tuple(smaps.shape)} does not match data ({B}, {C}) + {im}")
     sc = scaling_coef.to(device=grid.device, dtype=grid.dtype).contiguous()
-    out = torch.empty((B, 1 if smaps is not None else C) + im, dtype=grid.dtype, device=grid.device)
+    out = _out_or_new(out, (B, 1 if smaps is not None else C) + im, grid)
     with torch.cuda.device(grid.device):
         st = _lib.load().tkbn_crop_deapod(
             _lib.dtype_code(grid),
@@ -158,6 +158,14 @@ def fused_supported(dtype: torch.dtype, im_size, grid_size) -> bool:
     return ok
 
 
+def _out_or_new(out: Optional[Tensor], shape: tuple, like: Tensor) -> Tensor:
+    if out is None:
+        return torch.empty(shape, dtype=like.dtype, device=like.device)
+    if tuple(out.shape) != tuple(shape) or out.dtype != like.dtype or out.device != like.device or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous {tuple(shape)} {like.dtype} tensor on {like.device}")
+    return out
+
+
 def _fft_workspace(dev, d, im, gs, B, C):
     n = _lib.load().tkbn_fft_workspace_bytes(d, _lib.i64_array(im), _lib.i64_array(gs), B, C)
     if n < 0:
@@ -165,7 +173,8 @@ def _fft_workspace(dev, d, im, gs, B, C):
     return torch.empty(max(int(n), 1), dtype=torch.uint8, device=dev), int(n)
 
 
-def fft_grid_fwd(image: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_size, grid_size, scale: float) -> Tensor:
+def fft_grid_fwd(image: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_size, grid_size, scale: float,
+                 out: Optional[Tensor] = None) -> Tensor:
     """FFT(pad_end(image * smaps * scaling_coef)) * scale in one fused HIP pipeline."""
     require_gpu(image, smaps, scaling_coef)
     image = image.contiguous()
@@ -186,7 +195,7 @@ def fft_grid_fwd(image: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, i
         if image.shape[1] not in (1, C):
             raise ValueError("image coil dimension must be 1 or match smaps")
     sc = scaling_coef.to(device=image.device, dtype=image.dtype).contiguous()
-    grid = torch.empty((B, C) + gs, dtype=image.dtype, device=image.device)
+    grid = _out_or_new(out, (B, C) + gs, image)
     ws, nws = _fft_workspace(image.device, d, im, gs, B, C)
     with torch.cuda.device(image.device):
         st = _lib.load().tkbn_fft_grid_fwd(
@@ -198,7 +207,8 @@ def fft_grid_fwd(image: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, i
     return grid
 
 
-def fft_grid_adj(grid: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_size, scale: float) -> Tensor:
+def fft_grid_adj(grid: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_size, scale: float,
+                 out: Optional[Tensor] = None) -> Tensor:
     """sum_c crop(IFFT(grid)) * conj(scaling_coef) * conj(smaps_c) * scale (per coil without smaps)."""
     require_gpu(grid, smaps, scaling_coef)
     grid = grid.contiguous()
@@ -214,7 +224,7 @@ def fft_grid_adj(grid: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im
         if tuple(smaps.shape) != (B, C) + im:
             raise ValueError(f"smaps shape {tuple(smaps.shape)} does not match data ({B}, {C}) + {im}")
     sc = scaling_coef.to(device=grid.device, dtype=grid.dtype).contiguous()
-    out = torch.empty((B, 1 if smaps is not None else C) + im, dtype=grid.dtype, device=grid.device)
+    out = _out_or_new(out, (B, 1 if smaps is not None else C) + im, grid)
     ws, nws = _fft_workspace(grid.device, d, im, gs, B, C)
     with torch.cuda.device(grid.device):
         st = _lib.load().tkbn_fft_grid_adj(

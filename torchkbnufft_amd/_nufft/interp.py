@@ -113,7 +113,44 @@ def make_desc(
     desc.coils = coils
     desc.klength = omega.shape[-1]
     desc.omega_batched = 1 if omega.ndim == 3 else 0
+    desc.flags = 0
     return desc
+
+
+class PlanHolder:
+    """Caller-owned interpolation workspace (one per problem and direction).
+
+    The library keeps the trajectory plan in it.  With ``trusted = True`` the owner
+    promises that the trajectory, tables and geometry never change after the first
+    call through this holder, and later calls skip all plan work (TKBN_PLAN_REUSE);
+    otherwise the plan is checked on the device each call.  Use one stream per holder.
+    """
+
+    def __init__(self, trusted: bool = False):
+        self.ws: Optional[Tensor] = None
+        self.built = False
+        self.trusted = trusted
+
+    def get(self, like: Tensor, nbytes: int) -> Tensor:
+        if self.ws is None or self.ws.numel() < nbytes or self.ws.device != like.device:
+            self.ws = torch.empty(max(nbytes, _lib.PLAN_CACHE_HEADER_BYTES), dtype=torch.uint8, device=like.device)
+            self.ws[: _lib.PLAN_CACHE_HEADER_BYTES].zero_()
+            self.built = False
+        return self.ws
+
+    def flags(self) -> int:
+        return _lib.TKBN_PLAN_REUSE if (self.built and self.trusted) else 0
+
+
+def _check_out(out: Optional[Tensor], shape: tuple, like: Tensor) -> Tensor:
+    """``out`` if given (validated), else a new tensor of ``shape`` like ``like``."""
+    if out is None:
+        return torch.empty(shape, dtype=like.dtype, device=like.device)
+    if tuple(out.shape) != tuple(shape) or out.dtype != like.dtype or out.device != like.device:
+        raise ValueError(f"out must be a {tuple(shape)} {like.dtype} tensor on {like.device}")
+    if not out.is_contiguous():
+        raise ValueError("out must be contiguous")
+    return out
 
 
 def _prep_tables(tables: List[Tensor], like: Tensor) -> List[Tensor]:
@@ -134,8 +171,13 @@ def table_interp(
     table_oversamp: Tensor,
     offsets: Tensor,
     min_kspace_per_fork: int = 1024,
+    out: Optional[Tensor] = None,
+    plan: Optional[PlanHolder] = None,
 ) -> Tensor:
-    """Interpolate gridded ``image`` (B, C, *K) to ``omega``; returns (B, C, M)."""
+    """Interpolate gridded ``image`` (B, C, *K) to ``omega``; returns (B, C, M).
+
+    ``out``: optional contiguous (B, C, M) tensor to write into.
+    """
     omega = _check_omega(omega, image.shape[0], "image")
     require_gpu(image, omega)
     key_omega = omega
@@ -145,20 +187,26 @@ def table_interp(
     grid_size = tuple(image.shape[2:])
     B, C = image.shape[0], image.shape[1]
     desc = make_desc(image, omega, tables, n_shift, numpoints, table_oversamp, grid_size, B, C)
-    kdata = torch.empty(B, C, omega.shape[-1], dtype=image.dtype, device=image.device)
+    kdata = _check_out(out, (B, C, omega.shape[-1]), image)
     lib = _lib.load()
     ws, nws = None, 0
     # binned LDS-staged gather once there is enough work to amortise the plan launches
     if omega.shape[-1] * B * C >= _BINNED_FWD_MIN_WORK:
         nws = int(lib.tkbn_interp_fwd_workspace_bytes(ctypes.byref(desc)))
         if nws > 0:
-            ws = _plan_workspace("fwd", key_omega, image, nws)
+            if plan is not None:
+                ws = plan.get(image, nws)
+                desc.flags = plan.flags()
+            else:
+                ws = _plan_workspace("fwd", key_omega, image, nws)
     with torch.cuda.device(image.device):
         st = lib.tkbn_interp_fwd(
             ctypes.byref(desc), image.data_ptr(), omega.data_ptr(), kdata.data_ptr(),
             ws.data_ptr() if ws is not None else None, ws.numel() if ws is not None else 0, _lib.stream_of(image),
         )
     _lib.check(st, "tkbn_interp_fwd")
+    if plan is not None and ws is not None:
+        plan.built = True
     return kdata
 
 
@@ -172,8 +220,13 @@ def table_interp_adjoint(
     offsets: Tensor,
     grid_size,
     deterministic: Optional[bool] = None,
+    out: Optional[Tensor] = None,
+    plan: Optional[PlanHolder] = None,
 ) -> Tensor:
-    """Spread off-grid ``data`` (B, C, M) onto a grid of ``grid_size``; returns (B, C, *K)."""
+    """Spread off-grid ``data`` (B, C, M) onto a grid of ``grid_size``; returns (B, C, *K).
+
+    ``out``: optional contiguous (B, C, *K) tensor to write into (it is fully overwritten).
+    """
     omega = _check_omega(omega, data.shape[0], "data")
     require_gpu(data, omega)
     key_omega = omega
@@ -189,11 +242,15 @@ def table_interp_adjoint(
         _lib.TKBN_ADJ_DETERMINISTIC if deterministic else _lib.TKBN_ADJ_ATOMIC
     )
     lib = _lib.load()
-    grid = torch.empty((B, C) + gs, dtype=data.dtype, device=data.device)
+    grid = _check_out(out, (B, C) + gs, data)
     ws_bytes = lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(desc), mode)
     if ws_bytes < 0:
         _lib.check(_lib.TKBN_ERR_INVALID, "tkbn_interp_adj_workspace_bytes")
-    ws = _plan_workspace("adj", key_omega, data, int(ws_bytes))
+    if plan is not None:
+        ws = plan.get(data, int(ws_bytes))
+        desc.flags = plan.flags()
+    else:
+        ws = _plan_workspace("adj", key_omega, data, int(ws_bytes))
     with torch.cuda.device(data.device):
         st = lib.tkbn_interp_adj(
             ctypes.byref(desc),
@@ -206,6 +263,8 @@ def table_interp_adjoint(
             _lib.stream_of(data),
         )
     _lib.check(st, "tkbn_interp_adj")
+    if plan is not None:
+        plan.built = True
     return grid
 
 

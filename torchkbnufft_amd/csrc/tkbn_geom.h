@@ -173,6 +173,7 @@ inline int validate_desc(const tkbn_interp_desc* d) {
     G *= d->grid_size[i];
   }
   TKBN_REQUIRE(G < (1LL << 40), TKBN_ERR_INVALID, "grid too large");
+  TKBN_REQUIRE((d->flags & ~TKBN_PLAN_REUSE) == 0, TKBN_ERR_INVALID, "unknown flags 0x%x", d->flags);
   return TKBN_OK;
 }
 

@@ -130,6 +130,9 @@ inline int64_t plan_cache_bytes(const tkbn_interp_desc* d) {
   return kCacheHdrBytes + ((s.total * 4 + 255) / 256) * 256;
 }
 
+// True when the caller asserted that the workspace's plan is current (no plan work).
+inline bool plan_reused(const tkbn_interp_desc* d) { return (d->flags & TKBN_PLAN_REUSE) != 0; }
+
 // Enqueues begin + compare; returns the header (device) for the plan kernels.
 inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const void* omega, void* ws, hipStream_t s,
                             CacheHdr** hdr) {

@@ -1354,14 +1354,16 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
                   void* maxabs_slot, void* grid, size_t lds, hipStream_t s, const CacheHdr* cache) {
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t zdim = g.batched ? g.B : 1;
-  if (g.M > 0) {
+  if (cache && g.M > 0) {  // no header: the caller reuses the plan
     dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
     bin_count_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
     TKBN_LAUNCH_CHECK("bin_count_kernel");
   }
-  plan_scan_kernel<<<dim3((unsigned)zdim), dim3(1024), 0, s>>>(tg, ws, cache);
-  TKBN_LAUNCH_CHECK("plan_scan_kernel");
-  if (g.M > 0) {
+  if (cache) {
+    plan_scan_kernel<<<dim3((unsigned)zdim), dim3(1024), 0, s>>>(tg, ws, cache);
+    TKBN_LAUNCH_CHECK("plan_scan_kernel");
+  }
+  if (cache && g.M > 0) {
     dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
     bin_scatter_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
     TKBN_LAUNCH_CHECK("bin_scatter_kernel");
@@ -1479,25 +1481,28 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
                                void* grid, hipStream_t s) {
   const Geom<float> g = make_geom<float>(d);
   const TileGeom tg = make_coil_tile_geom(d);
-  CacheHdr* cache = nullptr;
-  int st = plan_cache_begin(d, kPlanSpreadCoils, omega, ws, s, &cache);
-  if (st != TKBN_OK) return st;
   int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
-  st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
-  if (st != TKBN_OK) return st;
-  dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
-  if (g.M > 0) {
-    coil_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-    TKBN_LAUNCH_CHECK("coil_bin_count_kernel");
+  int st = TKBN_OK;
+  if (!plan_reused(d)) {
+    CacheHdr* cache = nullptr;
+    st = plan_cache_begin(d, kPlanSpreadCoils, omega, ws, s, &cache);
+    if (st != TKBN_OK) return st;
+    st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
+    if (st != TKBN_OK) return st;
+    dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
+    if (g.M > 0) {
+      coil_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
+      TKBN_LAUNCH_CHECK("coil_bin_count_kernel");
+    }
+    coil_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
+    TKBN_LAUNCH_CHECK("coil_plan_scan_kernel");
+    if (g.M > 0) {
+      coil_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
+      TKBN_LAUNCH_CHECK("coil_bin_scatter_kernel");
+    }
+    st = plan_cache_commit(d, kPlanSpreadCoils, omega, ws, s);
+    if (st != TKBN_OK) return st;
   }
-  coil_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
-  TKBN_LAUNCH_CHECK("coil_plan_scan_kernel");
-  if (g.M > 0) {
-    coil_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-    TKBN_LAUNCH_CHECK("coil_bin_scatter_kernel");
-  }
-  st = plan_cache_commit(d, kPlanSpreadCoils, omega, ws, s);
-  if (st != TKBN_OK) return st;
   const int64_t nrows = g.B * g.C;
   const unsigned ychunks = (unsigned)ceil_div(nrows, 64);
   coil_zero_hot_kernel<<<dim3((unsigned)tg.NB, ychunks, 1), dim3(kRegT0 * kRegT1), 0, s>>>(
@@ -1531,17 +1536,23 @@ int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const v
   const size_t lds = spread_lds_bytes(d, tg, det);
   const int64_t nplans = d->omega_batched ? d->batch : 1;
   const int kind = kPlanSpreadTiles + (det ? 1 : 0);
+  const bool reuse = plan_reused(d);
   CacheHdr* cache = nullptr;
-  int st = plan_cache_begin(d, kind, omega, ws, s, &cache);
-  if (st != TKBN_OK) return st;
+  int st = TKBN_OK;
+  if (!reuse) {
+    st = plan_cache_begin(d, kind, omega, ws, s, &cache);
+    if (st != TKBN_OK) return st;
+  }
   int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
   void* slot = reinterpret_cast<char*>(wsi) + nplans * tg.plan_ints * 4;
-  // counts must start at zero: clear the plan area unless the cached plan is reused
-  st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
-  if (st != TKBN_OK) return st;
+  if (!reuse) {
+    // counts must start at zero: clear the plan area unless the cached plan is reused
+    st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
+    if (st != TKBN_OK) return st;
+  }
   st = spread_dispatch<T>(g, tg, det, kdata, omega, wsi, slot, grid, lds, s, cache, d->ndim, max_numpoints(d));
   if (st != TKBN_OK) return st;
-  return plan_cache_commit(d, kind, omega, ws, s);
+  return reuse ? TKBN_OK : plan_cache_commit(d, kind, omega, ws, s);
 }
 
 template <typename T, int D, int JM>
@@ -1549,13 +1560,15 @@ int fwd_tiled_launch(const Geom<T>& g, const TileGeom& tg, const void* grid, con
                      hipStream_t s, const CacheHdr* cache) {
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t zdim = g.batched ? g.B : 1;
-  dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
-  bin_count_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
-  TKBN_LAUNCH_CHECK("bin_count_kernel<own>");
-  plan_scan_kernel<<<dim3((unsigned)zdim), dim3(1024), 0, s>>>(tg, ws, cache);
-  TKBN_LAUNCH_CHECK("plan_scan_kernel<own>");
-  bin_scatter_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
-  TKBN_LAUNCH_CHECK("bin_scatter_kernel<own>");
+  if (cache) {  // no header: the caller reuses the plan
+    dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
+    bin_count_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
+    TKBN_LAUNCH_CHECK("bin_count_kernel<own>");
+    plan_scan_kernel<<<dim3((unsigned)zdim), dim3(1024), 0, s>>>(tg, ws, cache);
+    TKBN_LAUNCH_CHECK("plan_scan_kernel<own>");
+    bin_scatter_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
+    TKBN_LAUNCH_CHECK("bin_scatter_kernel<own>");
+  }
   const size_t lds = (size_t)tg.CC * tg.rcells * sizeof(cplx<T>);
   interp_fwd_tiled_kernel<T, D, JM>
       <<<dim3((unsigned)tg.maxwork, (unsigned)ceil_div(nrows, tg.CC), (unsigned)zdim), dim3(256), lds, s>>>(
@@ -1576,15 +1589,19 @@ int interp_fwd_tiled_t(const tkbn_interp_desc* d, const void* grid, const void* 
   const Geom<T> g = make_geom<T>(d);
   const TileGeom tg = make_tile_geom(d, false, true);
   const int64_t nplans = d->omega_batched ? d->batch : 1;
+  const bool reuse = plan_reused(d);
   CacheHdr* cache = nullptr;
-  int st = plan_cache_begin(d, kPlanFwdTiles, omega, ws, s, &cache);
-  if (st != TKBN_OK) return st;
+  int st = TKBN_OK;
   int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
-  st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
-  if (st != TKBN_OK) return st;
+  if (!reuse) {
+    st = plan_cache_begin(d, kPlanFwdTiles, omega, ws, s, &cache);
+    if (st != TKBN_OK) return st;
+    st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
+    if (st != TKBN_OK) return st;
+  }
   st = fwd_tiled_dispatch<T>(g, tg, grid, omega, wsi, kdata, s, cache, d->ndim, max_numpoints(d));
   if (st != TKBN_OK) return st;
-  return plan_cache_commit(d, kPlanFwdTiles, omega, ws, s);
+  return reuse ? TKBN_OK : plan_cache_commit(d, kPlanFwdTiles, omega, ws, s);
 }
 
 // ---- multi-coil forward gather ---------------------------------------------------
@@ -1639,21 +1656,24 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
                                hipStream_t s) {
   const Geom<float> g = make_geom<float>(d);
   const TileGeom tg = make_gather_geom(d);
-  CacheHdr* cache = nullptr;
-  int st = plan_cache_begin(d, kPlanGatherCoils, omega, ws, s, &cache);
-  if (st != TKBN_OK) return st;
   int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
-  st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
-  if (st != TKBN_OK) return st;
-  dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
-  gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-  TKBN_LAUNCH_CHECK("gat_bin_count_kernel");
-  gat_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
-  TKBN_LAUNCH_CHECK("gat_plan_scan_kernel");
-  gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-  TKBN_LAUNCH_CHECK("gat_bin_scatter_kernel");
-  st = plan_cache_commit(d, kPlanGatherCoils, omega, ws, s);
-  if (st != TKBN_OK) return st;
+  int st = TKBN_OK;
+  if (!plan_reused(d)) {
+    CacheHdr* cache = nullptr;
+    st = plan_cache_begin(d, kPlanGatherCoils, omega, ws, s, &cache);
+    if (st != TKBN_OK) return st;
+    st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
+    if (st != TKBN_OK) return st;
+    dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
+    gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
+    TKBN_LAUNCH_CHECK("gat_bin_count_kernel");
+    gat_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
+    TKBN_LAUNCH_CHECK("gat_plan_scan_kernel");
+    gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
+    TKBN_LAUNCH_CHECK("gat_bin_scatter_kernel");
+    st = plan_cache_commit(d, kPlanGatherCoils, omega, ws, s);
+    if (st != TKBN_OK) return st;
+  }
   const unsigned ychunks = (unsigned)ceil_div(g.B * g.C, 64);
   const int64_t groups = ceil_div(tg.maxwork, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
   profile_before(s);
