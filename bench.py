@@ -166,6 +166,120 @@ def cpu_baseline(x, ktraj, smaps, seconds: float) -> dict:
     }
 
 
+def vd_spiral(n_inter: int, n_read: int, turns: float = 16.0) -> torch.Tensor:
+    """Variable-density spiral, r(t) = pi * t^1.5 (t = i / n_read), theta(t) = 2 pi turns t
+    + 2 pi l / n_inter; rows (ky, kx); radius kept just below pi."""
+    t = np.arange(n_read) / n_read
+    r = np.pi * t**1.5 * 0.9999
+    arms = []
+    for l in range(n_inter):
+        th = 2 * np.pi * turns * t + 2 * np.pi * l / n_inter
+        arms.append(np.stack([r * np.sin(th), r * np.cos(th)]))
+    return torch.tensor(np.concatenate(arms, axis=1), dtype=torch.float32)
+
+
+def timed(fn, iters: int) -> float:
+    """Mean ms per call of fn (synchronised wall clock around `iters` calls)."""
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / iters * 1e3
+
+
+def bench_batched(dev, iters: int) -> dict:
+    """64 uneven problems: image N in {128, 192, 256}^2, 8 coils, VD spiral of 8-48
+    interleaves x 1024 samples; BatchedKbNufft (graph replay) vs the plain per-problem
+    loop of KbNufft / KbNufftAdjoint calls."""
+    import torchkbnufft_amd as tkbn
+
+    gen = torch.Generator().manual_seed(3)
+    nprob, coils = 64, 8
+    sizes = [(int(s), int(s)) for s in torch.tensor([128, 192, 256])[torch.randint(0, 3, (nprob,), generator=gen)]]
+    inter = torch.randint(8, 49, (nprob,), generator=gen).tolist()
+    ktrajs = [vd_spiral(n, 1024) for n in inter]
+    images = [torch.complex(torch.randn((1, coils) + s, generator=gen), torch.randn((1, coils) + s, generator=gen)).to(dev)
+              for s in sizes]
+    kt = [k.to(dev) for k in ktrajs]
+    ops = {s: (tkbn.KbNufft(im_size=s, dtype=torch.complex64).to(dev), tkbn.KbNufftAdjoint(im_size=s, dtype=torch.complex64).to(dev))
+           for s in sorted(set(sizes))}
+
+    def loop():
+        return [ops[s][1](ops[s][0](x, k), k) for s, x, k in zip(sizes, images, kt)]
+
+    t_loop = timed(loop, iters)
+    ref = loop()
+    bt = tkbn.BatchedKbNufft(sizes, ktrajs, coils=coils, device=dev)
+    for p, x in enumerate(images):
+        bt.images[p].copy_(x)
+    t_eager = timed(bt.pair, iters)
+    bt.capture()
+    t_graph = timed(bt.replay, iters)
+    out = bt.replay()
+    torch.cuda.synchronize()
+    diff = max(float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in zip(out, ref))
+    return {
+        "workload": "64 problems, N in {128,192,256}^2 (uniform), 8 coils, VD spiral 8-48 interleaves x 1024 "
+        "(r = pi t^1.5, 16 turns), fwd + adj of every problem per step",
+        "samples": int(sum(k.shape[1] for k in ktrajs)),
+        "plain_loop_ms": t_loop,
+        "batched_eager_ms": t_eager,
+        "batched_graph_ms": t_graph,
+        "speedup_graph_vs_loop": t_loop / t_graph,
+        "max_rel_diff_vs_loop": diff,
+    }
+
+
+def bench_grad(dev, iters: int) -> dict:
+    """Gradients of ||A(x, ktraj) - y||^2 w.r.t. the image and the trajectory: 256^2, 12
+    coils as SENSE maps, 64-interleave VD spiral x 1536 (98304 samples); one SENSE
+    operator vs the plain loop of 12 single-coil operators (same gradients)."""
+    import torchkbnufft_amd as tkbn
+
+    gen = torch.Generator().manual_seed(4)
+    n, coils = 256, 12
+    ktraj = vd_spiral(64, 1536).to(dev)
+    s = torch.complex(torch.randn(1, coils, n, n, generator=gen), torch.randn(1, coils, n, n, generator=gen)).to(dev)
+    f = torch.fft.fftfreq(n, device=dev)
+    g1 = torch.exp(-2 * (math.pi**2) * (20.0**2) * f**2)
+    s = torch.fft.ifft2(torch.fft.fft2(s) * (g1[:, None] * g1[None, :]))
+    smaps = (s / torch.sqrt(torch.sum(torch.abs(s) ** 2, dim=1, keepdim=True))).to(torch.complex64).contiguous()
+    fw = tkbn.KbNufft(im_size=(n, n), dtype=torch.complex64).to(dev)
+    x_true = torch.complex(torch.randn(1, 1, n, n, generator=gen), torch.randn(1, 1, n, n, generator=gen)).to(dev)
+    with torch.no_grad():
+        y = fw(x_true, ktraj, smaps=smaps)
+        y = y + 0.01 * torch.complex(torch.randn(y.shape, generator=gen), torch.randn(y.shape, generator=gen)).to(dev)
+    x0 = torch.complex(torch.randn(1, 1, n, n, generator=gen), torch.randn(1, 1, n, n, generator=gen)).to(dev)
+
+    def grads(per_coil: bool):
+        x = x0.clone().requires_grad_(True)
+        k = ktraj.clone().requires_grad_(True)
+        if per_coil:
+            loss = sum(torch.sum(torch.abs(fw(x, k, smaps=smaps[:, c : c + 1]) - y[:, c : c + 1]) ** 2)
+                       for c in range(coils))
+        else:
+            loss = torch.sum(torch.abs(fw(x, k, smaps=smaps) - y) ** 2)
+        loss.backward()
+        return x.grad, k.grad
+
+    t_fused = timed(lambda: grads(False), iters)
+    t_loop = timed(lambda: grads(True), iters)
+    gx, gk = grads(False)
+    lx, lk = grads(True)
+    torch.cuda.synchronize()
+    return {
+        "workload": "256^2, 12 coils (SENSE maps), 64-interleave VD spiral x 1536 (98304 samples); "
+        "loss ||A(x, ktraj) - y||^2, y = A(x_true) + N(0, 0.01); d/dx and d/dktraj",
+        "fused_ms": t_fused,
+        "per_coil_loop_ms": t_loop,
+        "speedup_vs_loop": t_loop / t_fused,
+        "rel_diff_dx": float(torch.linalg.norm(gx - lx) / torch.linalg.norm(lx)),
+        "rel_diff_dktraj": float(torch.linalg.norm(gk - lk) / torch.linalg.norm(lk)),
+    }
+
+
 def fp64_oracle_error(fw, x, ktraj, smaps) -> float:
     """Forward NUFFT vs FP64 NUDFT on a 1-coil slice and a sample subset."""
     from oracle import nudft
@@ -185,6 +299,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-iters", type=int, default=20)
+    ap.add_argument("--form-iters", type=int, default=5, help="iterations of the batched / gradient forms")
+    ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -273,6 +389,9 @@ def main() -> int:
         "stage_bytes": sb,
         "fp64_oracle_rel_err": fp64_oracle_error(fw, x, ktraj, smaps),
     }
+    if not args.no_forms:
+        out["batched"] = bench_batched(dev, args.form_iters)
+        out["grad"] = bench_grad(dev, args.form_iters)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, ktraj, smaps, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

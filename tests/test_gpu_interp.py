@@ -341,3 +341,29 @@ def test_adjoint_nonfinite_kdata_propagates(rows, deterministic):
     ref = _interp.table_interp_adjoint(y2.to(DEV), *args, deterministic=deterministic).cpu()
     keep = ~nan
     assert rel_l2(out[keep], ref[keep]) < 1e-6
+
+
+def test_plan_cache_key_covers_tile_shape():
+    """One trajectory through the module cache with many and with one coil row: the
+    sample-lane tile shape depends on the row count, so the cached plan must not be
+    shared between them."""
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    print("[case] plan cache across row counts", flush=True)
+    gen = torch.Generator().manual_seed(47)
+    grid, M = (256, 256), 30000
+    om = create_ktraj(2, M, gen, torch.float32).to(DEV)
+    fw = tkbn.KbInterp(im_size=(128, 128), grid_size=grid, dtype=torch.complex64).to(DEV)
+    ad = tkbn.KbInterpAdjoint(im_size=(128, 128), grid_size=grid, dtype=torch.complex64).to(DEV)
+    _interp.clear_plan_cache()
+    results = []
+    for rows in (12, 1, 12, 1):
+        x = gi.cplx((1, rows) + grid, gen).to(DEV)
+        y = gi.cplx((1, rows, M), gen).to(DEV)
+        results.append((rows, x, y, fw(x, om), ad(y, om)))
+    torch.cuda.synchronize()
+    for rows, x, y, f, a in results:
+        _interp.clear_plan_cache()
+        assert rel_l2(f, fw(x, om)) < 1e-6, rows
+        _interp.clear_plan_cache()
+        assert rel_l2(a, ad(y, om)) < 1e-6, rows

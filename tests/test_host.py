@@ -158,3 +158,19 @@ def test_math_helpers():
     assert torch.allclose(torch.view_as_complex(tkbn.complex_mult(ar, br)), a * b)
     assert torch.allclose(torch.view_as_complex(tkbn.conj_complex_mult(ar, br)), a * b.conj())
     assert torch.allclose(tkbn.imag_exp(torch.tensor([0.5])), torch.exp(1j * torch.tensor([0.5])))
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("rows", [1, 8, 64])
+def test_workspace_size_queries_are_host_only_and_positive(ndim, rows):
+    """Size queries run host arithmetic only (no GPU): every geometry gives a size."""
+    lib = _lib.load()
+    d = _desc(ndim=ndim)
+    d.coils, d.klength = rows, 1000
+    for i in range(3):
+        d.grid_size[i] = 64 if ndim < 3 else 24
+    assert lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), _lib.TKBN_ADJ_ATOMIC) > _lib.PLAN_CACHE_HEADER_BYTES
+    assert lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), _lib.TKBN_ADJ_DETERMINISTIC) > 0
+    assert lib.tkbn_interp_fwd_workspace_bytes(ctypes.byref(d)) >= 0
+    d.flags = 2
+    assert lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), 0) == -1  # unknown flag

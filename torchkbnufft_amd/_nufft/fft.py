@@ -87,7 +87,7 @@ def crop_deapod(grid: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im_
         if tuple(smaps.shape) != (B, C) + im:
             raise ValueError(f"smaps shape {tuple(smaps.shape)} does not match data ({B}, {C}) + {im}")
     sc = scaling_coef.to(device=grid.device, dtype=grid.dtype).contiguous()
-    out = _out_or_new(out, (B, 1 if smaps is not None else C) + im, grid)
+    out = torch.empty((B, 1 if smaps is not None else C) + im, dtype=grid.dtype, device=grid.device)
     with torch.cuda.device(grid.device):
         st = _lib.load().tkbn_crop_deapod(
             _lib.dtype_code(grid),

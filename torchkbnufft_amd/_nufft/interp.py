@@ -28,7 +28,8 @@ _BINNED_FWD_MIN_WORK = 1 << 16
 # (checked on the device).  Keyed by (kind, device, stream, trajectory tensor); the
 # key only picks a buffer, correctness never depends on it.
 _WORKSPACES: "OrderedDict[tuple, torch.Tensor]" = OrderedDict()
-_WORKSPACES_MAX = 4
+_WORKSPACES_MAX = 256  # entries
+_WORKSPACES_MAX_BYTES = 2 << 30  # total bytes held
 
 
 def _plan_workspace(kind: str, omega: Tensor, like: Tensor, nbytes: int) -> Tensor:
@@ -38,7 +39,9 @@ def _plan_workspace(kind: str, omega: Tensor, like: Tensor, nbytes: int) -> Tens
         ws = torch.empty(max(nbytes, _lib.PLAN_CACHE_HEADER_BYTES), dtype=torch.uint8, device=like.device)
         ws[: _lib.PLAN_CACHE_HEADER_BYTES].zero_()  # a zero header never matches: first call builds
         _WORKSPACES[key] = ws
-        while len(_WORKSPACES) > _WORKSPACES_MAX:
+        while len(_WORKSPACES) > 1 and (
+            len(_WORKSPACES) > _WORKSPACES_MAX or sum(w.numel() for w in _WORKSPACES.values()) > _WORKSPACES_MAX_BYTES
+        ):
             _WORKSPACES.popitem(last=False)
     else:
         _WORKSPACES.move_to_end(key)

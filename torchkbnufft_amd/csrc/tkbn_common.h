@@ -103,7 +103,7 @@ struct FastDiv {
   uint32_t d, m, s;
 };
 
-inline FastDiv make_fastdiv(uint32_t d) {
+inline FastDiv make_fastdiv(uint32_t d) {  // d >= 1
   uint32_t s = 0;
   while (s < 31 && (1ull << s) < d) ++s;
   const uint64_t m = ((((uint64_t)1 << 32) * (((uint64_t)1 << s) - d)) / d) + 1;

@@ -93,11 +93,14 @@ inline unsigned long long fnv1a(unsigned long long h, const void* data, size_t n
 }
 
 // Fingerprint of what the plan depends on besides the compared words: the plan kind,
-// dtype, dimensions, grid, numpoints, oversampling, n_shift and sample count (not the
-// batch or coil count, which the plan does not depend on).
-inline unsigned long long plan_cache_key(const tkbn_interp_desc* d, int kind) {
+// its tile shape (which can depend on the coil count), dtype, dimensions, grid,
+// numpoints, oversampling, n_shift, sample count and the number of per-batch plans.
+inline unsigned long long plan_cache_key(const tkbn_interp_desc* d, int kind, const int* tile) {
   unsigned long long h = 1469598103934665603ULL;
   h = fnv1a(h, &kind, sizeof(kind));
+  h = fnv1a(h, tile, 3 * sizeof(int));
+  const int64_t nplans = d->omega_batched ? d->batch : 1;
+  h = fnv1a(h, &nplans, sizeof(nplans));
   h = fnv1a(h, &d->dtype, sizeof(d->dtype));
   h = fnv1a(h, &d->ndim, sizeof(d->ndim));
   h = fnv1a(h, d->grid_size, sizeof(d->grid_size));
@@ -134,12 +137,12 @@ inline int64_t plan_cache_bytes(const tkbn_interp_desc* d) {
 inline bool plan_reused(const tkbn_interp_desc* d) { return (d->flags & TKBN_PLAN_REUSE) != 0; }
 
 // Enqueues begin + compare; returns the header (device) for the plan kernels.
-inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const void* omega, void* ws, hipStream_t s,
-                            CacheHdr** hdr) {
+inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const int* tile, const void* omega, void* ws,
+                            hipStream_t s, CacheHdr** hdr) {
   CacheHdr* h = static_cast<CacheHdr*>(ws);
   const CacheSrc src = plan_cache_src(d, omega);
   const unsigned int* copy = reinterpret_cast<const unsigned int*>(static_cast<char*>(ws) + kCacheHdrBytes);
-  cache_begin_kernel<<<1, 64, 0, s>>>(h, plan_cache_key(d, kind));
+  cache_begin_kernel<<<1, 64, 0, s>>>(h, plan_cache_key(d, kind, tile));
   TKBN_LAUNCH_CHECK("cache_begin_kernel");
   const int64_t blocks = std::min<int64_t>(ceil_div(src.total, 256), 1024);
   if (blocks > 0) {
@@ -150,12 +153,13 @@ inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const void* ome
   return TKBN_OK;
 }
 
-inline int plan_cache_commit(const tkbn_interp_desc* d, int kind, const void* omega, void* ws, hipStream_t s) {
+inline int plan_cache_commit(const tkbn_interp_desc* d, int kind, const int* tile, const void* omega, void* ws,
+                             hipStream_t s) {
   CacheHdr* h = static_cast<CacheHdr*>(ws);
   const CacheSrc src = plan_cache_src(d, omega);
   unsigned int* copy = reinterpret_cast<unsigned int*>(static_cast<char*>(ws) + kCacheHdrBytes);
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(src.total, 256), 1024));
-  cache_commit_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(h, src, copy, plan_cache_key(d, kind));
+  cache_commit_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(h, src, copy, plan_cache_key(d, kind, tile));
   TKBN_LAUNCH_CHECK("cache_commit_kernel");
   return TKBN_OK;
 }

@@ -50,6 +50,7 @@ struct TileGeom {
   // workspace offsets (in int32 units) of one trajectory's plan
   int64_t o_key, o_perm, o_count, o_start, o_cursor, o_wstart, o_work, o_nwork, plan_ints;
   int64_t o_maxabs_bytes;  // byte offset of the max|kdata| slot (8 bytes)
+  FastDiv fT[3], fl[3], fR[3], ftc, frc;  // dividers by T, lstride, R, tcells, rcells
   int64_t nkeys;           // multi-coil plan: (tile, footprint column) keys
   int64_t o_hot, o_cold;   // multi-coil plan: scans of split / unsplit tiles
   int64_t o_rec_bytes;     // multi-coil plan: byte offset of the entry records
@@ -290,12 +291,17 @@ __global__ void __launch_bounds__(256) maxabs_kernel(const T* __restrict__ x, in
   }
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nom; i += (int64_t)gridDim.x * blockDim.x)
     if (!isfinite(om[i])) mx = (T)INFINITY;
-  // wave reduce then one atomic per wave
+  // wave reduce, block reduce in LDS, then one atomic per block (few blocks: the
+  // atomics all hit one address)
   for (int off = 32; off > 0; off >>= 1) {
     const T o = __shfl_down(mx, off, 64);
     mx = o > mx ? o : mx;
   }
-  if ((threadIdx.x & 63) == 0) {
+  __shared__ T wmax[4];  // 256 threads = 4 waves
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mx = wmax[w] > mx ? wmax[w] : mx;
     U bits;
     memcpy(&bits, &mx, sizeof(T));
     atomicMax(static_cast<U*>(slot), bits);  // non-negative floats order like their bits
@@ -323,7 +329,7 @@ __global__ void __launch_bounds__(256)
     int64_t gi = 0;
     bool in = true;
     for (int d = 0; d < D; ++d) {
-      const int l = (cell / tg.lstride[d]) % tg.T[d];
+      const int l = (int)fmodd(fdiv(cell, tg.fl[d]), tg.fT[d]);
       const int k = origin[d] + l;
       in = in && k < g.K[d];
       gi += (int64_t)k * g.stride[d];
@@ -478,13 +484,13 @@ __global__ void __launch_bounds__(256)
   // store the owned cells
   const double inv = 1.0 / scale;
   for (int idx = threadIdx.x; idx < ncc * tg.tcells; idx += blockDim.x) {
-    const int c = idx / tg.tcells;
+    const int c = (int)fdiv(idx, tg.ftc);
     const int cell = idx - c * tg.tcells;
     int64_t gi = 0;
     bool in = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int l = (cell / tg.lstride[d]) % tg.T[d];
+      const int l = (int)fmodd(fdiv(cell, tg.fl[d]), tg.fT[d]);
       const int kk = origin[d] + l;
       in = in && kk < g.K[d];
       gi += (int64_t)kk * g.stride[d];
@@ -1178,13 +1184,14 @@ __global__ void __launch_bounds__(256)
   // stage: lds[c * rcells + cell] = grid[row c][(origin + l) mod K]
   const int rc = tg.rcells;
   for (int idx = threadIdx.x; idx < ncc * rc; idx += blockDim.x) {
-    const int c = idx / rc;
+    const int c = (int)fdiv(idx, tg.frc);
     int cell = idx - c * rc;
     int64_t gi = 0;
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const int l = cell % tg.R[d];
-      cell /= tg.R[d];
+      const int q = (int)fdiv(cell, tg.fR[d]);
+      const int l = cell - q * tg.R[d];
+      cell = q;
       int k = origin[d] + l;
       while (k >= g.K[d]) k -= g.K[d];
       gi += (int64_t)k * g.stride[d];
@@ -1257,11 +1264,41 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// Tile shapes tried in order (largest first): the first whose LDS staging holds
+// min(rows, 8) coil rows at once is used, so a sample's weights are evaluated once
+// per tile for small coil counts; large row counts take the first shape.
+static const int* tile_shape(int D, bool fwd, int64_t nrows, int64_t numpoints_max, bool f32) {
+  static const int spread2[2][3] = {{32, 32, 1}, {16, 16, 1}};
+  static const int spread3[3][3] = {{16, 16, 8}, {8, 8, 8}, {8, 8, 4}};
+  static const int fwd2[2][3] = {{32, 32, 1}, {16, 16, 1}};
+  static const int fwd3[1][3] = {{8, 8, 8}};
+  static const int one[1][3] = {{1024, 1, 1}};
+  const int(*cand)[3] = one;
+  int n = 1;
+  if (D == 2) {
+    cand = fwd ? fwd2 : spread2;
+    n = 2;
+  } else if (D == 3) {
+    cand = fwd ? fwd3 : spread3;
+    n = fwd ? 1 : 3;
+  }
+  const int64_t want = nrows < 8 ? (nrows > 0 ? nrows : 1) : 8;
+  for (int c = 0; c < n; ++c) {
+    int64_t cells = 1;
+    for (int i = 0; i < D; ++i) cells *= fwd ? cand[c][i] + numpoints_max - 1 : cand[c][i];
+    const int64_t per_row = fwd ? cells * (f32 ? 8 : 16) : 2 * cells * 8;
+    if (per_row * want <= kLdsBudget) return cand[c];
+  }
+  return cand[n - 1];
+}
+
 static TileGeom make_tile_geom(const tkbn_interp_desc* d, bool det, bool fwd = false) {
   TileGeom tg{};
   const int D = d->ndim;
-  static const int defaults[3][3] = {{1024, 1, 1}, {32, 32, 1}, {16, 16, 8}};
-  static const int fwd_defaults[3][3] = {{1024, 1, 1}, {32, 32, 1}, {8, 8, 8}};
+  const int64_t rows0 = d->omega_batched ? d->coils : d->batch * d->coils;
+  int64_t jmax = 1;
+  for (int i = 0; i < D; ++i) jmax = d->numpoints[i] > jmax ? d->numpoints[i] : jmax;
+  const int* shape = tile_shape(D, fwd, rows0, jmax, d->dtype == TKBN_F32);
   int tcells = 1;
   for (int i = 0; i < 3; ++i) {
     tg.T[i] = 1;
@@ -1270,7 +1307,7 @@ static TileGeom make_tile_geom(const tkbn_interp_desc* d, bool det, bool fwd = f
     tg.J[i] = 1;
   }
   for (int i = 0; i < D; ++i) {
-    int T = fwd ? fwd_defaults[D - 1][i] : defaults[D - 1][i];
+    int T = shape[i];
     const int need = (int)d->numpoints[i] - 1;
     if (T < need) T = need;
     if (T > d->grid_size[i]) T = (int)d->grid_size[i];
@@ -1335,6 +1372,14 @@ static TileGeom make_tile_geom(const tkbn_interp_desc* d, bool det, bool fwd = f
   tg.o_nwork = o; o += 1;
   o = (o + 3) & ~3LL;  // 16-byte align each plan
   tg.plan_ints = o;
+  // dimensions >= D keep extent 1 (their strides are unused and may be 0)
+  for (int i = 0; i < 3; ++i) {
+    tg.fT[i] = make_fastdiv((uint32_t)std::max(tg.T[i], 1));
+    tg.fl[i] = make_fastdiv((uint32_t)std::max(tg.lstride[i], 1));
+    tg.fR[i] = make_fastdiv((uint32_t)std::max(tg.R[i], 1));
+  }
+  tg.ftc = make_fastdiv((uint32_t)std::max(tg.tcells, 1));
+  tg.frc = make_fastdiv((uint32_t)std::max(tg.rcells, 1));
   return tg;
 }
 
@@ -1384,7 +1429,7 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
     if (st != TKBN_OK) return st;
     const int64_t n = g.B * g.C * g.M * 2;
     const int64_t nom = (g.batched ? g.B : 1) * D * g.M;
-    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(std::max(n, nom), 256), 2048));
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(std::max(n, nom), 1024), 256));
     maxabs_kernel<T><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(static_cast<const T*>(kdata), n, om, nom,
                                                                    maxabs_slot);
     TKBN_LAUNCH_CHECK("maxabs_kernel");
@@ -1485,7 +1530,7 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
   int st = TKBN_OK;
   if (!plan_reused(d)) {
     CacheHdr* cache = nullptr;
-    st = plan_cache_begin(d, kPlanSpreadCoils, omega, ws, s, &cache);
+    st = plan_cache_begin(d, kPlanSpreadCoils, tg.T, omega, ws, s, &cache);
     if (st != TKBN_OK) return st;
     st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
     if (st != TKBN_OK) return st;
@@ -1500,7 +1545,7 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
       coil_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
       TKBN_LAUNCH_CHECK("coil_bin_scatter_kernel");
     }
-    st = plan_cache_commit(d, kPlanSpreadCoils, omega, ws, s);
+    st = plan_cache_commit(d, kPlanSpreadCoils, tg.T, omega, ws, s);
     if (st != TKBN_OK) return st;
   }
   const int64_t nrows = g.B * g.C;
@@ -1540,7 +1585,7 @@ int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const v
   CacheHdr* cache = nullptr;
   int st = TKBN_OK;
   if (!reuse) {
-    st = plan_cache_begin(d, kind, omega, ws, s, &cache);
+    st = plan_cache_begin(d, kind, tg.T, omega, ws, s, &cache);
     if (st != TKBN_OK) return st;
   }
   int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
@@ -1552,7 +1597,7 @@ int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const v
   }
   st = spread_dispatch<T>(g, tg, det, kdata, omega, wsi, slot, grid, lds, s, cache, d->ndim, max_numpoints(d));
   if (st != TKBN_OK) return st;
-  return reuse ? TKBN_OK : plan_cache_commit(d, kind, omega, ws, s);
+  return reuse ? TKBN_OK : plan_cache_commit(d, kind, tg.T, omega, ws, s);
 }
 
 template <typename T, int D, int JM>
@@ -1594,14 +1639,14 @@ int interp_fwd_tiled_t(const tkbn_interp_desc* d, const void* grid, const void* 
   int st = TKBN_OK;
   int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
   if (!reuse) {
-    st = plan_cache_begin(d, kPlanFwdTiles, omega, ws, s, &cache);
+    st = plan_cache_begin(d, kPlanFwdTiles, tg.T, omega, ws, s, &cache);
     if (st != TKBN_OK) return st;
     st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
     if (st != TKBN_OK) return st;
   }
   st = fwd_tiled_dispatch<T>(g, tg, grid, omega, wsi, kdata, s, cache, d->ndim, max_numpoints(d));
   if (st != TKBN_OK) return st;
-  return reuse ? TKBN_OK : plan_cache_commit(d, kPlanFwdTiles, omega, ws, s);
+  return reuse ? TKBN_OK : plan_cache_commit(d, kPlanFwdTiles, tg.T, omega, ws, s);
 }
 
 // ---- multi-coil forward gather ---------------------------------------------------
@@ -1660,7 +1705,7 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
   int st = TKBN_OK;
   if (!plan_reused(d)) {
     CacheHdr* cache = nullptr;
-    st = plan_cache_begin(d, kPlanGatherCoils, omega, ws, s, &cache);
+    st = plan_cache_begin(d, kPlanGatherCoils, tg.T, omega, ws, s, &cache);
     if (st != TKBN_OK) return st;
     st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
     if (st != TKBN_OK) return st;
@@ -1671,7 +1716,7 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
     TKBN_LAUNCH_CHECK("gat_plan_scan_kernel");
     gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
     TKBN_LAUNCH_CHECK("gat_bin_scatter_kernel");
-    st = plan_cache_commit(d, kPlanGatherCoils, omega, ws, s);
+    st = plan_cache_commit(d, kPlanGatherCoils, tg.T, omega, ws, s);
     if (st != TKBN_OK) return st;
   }
   const unsigned ychunks = (unsigned)ceil_div(g.B * g.C, 64);
