@@ -280,6 +280,67 @@ def bench_grad(dev, iters: int) -> dict:
     }
 
 
+def bench_kooshball(dev, steps: int, warmup: int) -> dict:
+    """3-D radial kooshball: 160^3 -> 320^3, 8 coils with Gaussian-smoothed maps, 3200
+    spokes uniform on the sphere x 512 samples (1 638 400), deterministic spread;
+    forward + adjoint per step, plus the image and trajectory gradient of ||A x||^2."""
+    import torchkbnufft_amd as tkbn
+    from oracle import nudft
+
+    gen = torch.Generator().manual_seed(5)
+    n, coils, nsp, nread = 160, 8, 3200, 512
+    d = torch.randn(3, nsp, generator=gen, dtype=torch.float64)
+    d = d / torch.linalg.norm(d, dim=0, keepdim=True)
+    r = -np.pi + 2 * np.pi * torch.arange(nread, dtype=torch.float64) / nread
+    ktraj = (d[:, :, None] * r[None, None, :]).reshape(3, -1).to(torch.float32).to(dev)
+    x = torch.complex(torch.randn(1, 1, n, n, n, generator=gen), torch.randn(1, 1, n, n, n, generator=gen)).to(dev)
+    s = torch.complex(torch.randn(1, coils, n, n, n, generator=gen), torch.randn(1, coils, n, n, n, generator=gen)).to(dev)
+    f = torch.fft.fftfreq(n, device=dev)
+    g1 = torch.exp(-2 * (math.pi**2) * (20.0**2) * f**2)
+    s = torch.fft.ifftn(torch.fft.fftn(s, dim=(-3, -2, -1)) * (g1[:, None, None] * g1[None, :, None] * g1[None, None, :]),
+                        dim=(-3, -2, -1))
+    smaps = (s / torch.sqrt(torch.sum(torch.abs(s) ** 2, dim=1, keepdim=True))).to(torch.complex64).contiguous()
+    del s
+    fw = tkbn.KbNufft(im_size=(n, n, n), dtype=torch.complex64).to(dev)
+    ad = tkbn.KbNufftAdjoint(im_size=(n, n, n), dtype=torch.complex64).to(dev)
+    was = tkbn.deterministic_enabled()
+    tkbn.set_deterministic(True)
+    try:
+        def pair():
+            return ad(fw(x, ktraj, smaps=smaps), ktraj, smaps=smaps)
+
+        t_pair = timed(pair, steps) if warmup >= 0 else None
+        a, b = pair(), pair()
+        repeat = bool(torch.equal(a, b))
+
+        def grad():
+            xx = x.clone().requires_grad_(True)
+            kk = ktraj.clone().requires_grad_(True)
+            torch.sum(torch.abs(fw(xx, kk, smaps=smaps)) ** 2).backward()
+            return xx.grad, kk.grad
+
+        t_grad = timed(grad, max(1, steps // 2))
+        sel = torch.arange(0, ktraj.shape[1], ktraj.shape[1] // 16, device=dev)[:16]  # 16 x 4.1M-term NUDFT
+        y = fw(x, ktraj, smaps=smaps[:, :1])[..., sel].cpu().numpy()
+        xs = (x * smaps[:, :1]).cpu().numpy().astype(np.complex128)
+        ref = nudft.nudft(xs, ktraj[:, sel].double().cpu().numpy(), (n // 2, n // 2, n // 2))
+        err = float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+    finally:
+        tkbn.set_deterministic(was)
+    return {
+        "metric": "3-D kooshball fwd+adj NUFFT/s (deterministic)",
+        "value": 1e3 / t_pair,
+        "unit": "pairs/s",
+        "ms_per_pair": t_pair,
+        "grad_ms": t_grad,
+        "adjoint_bitwise_repeatable": repeat,
+        "fp64_oracle_rel_err": err,
+        "config": {"workload": "kooshball-3d: 160^3 -> 320^3, 8 coils (smoothed SENSE maps), 3200 spokes x 512 "
+                               "(1638400 samples), numpoints 6, deterministic adjoint", "coils": coils,
+                   "klength": int(ktraj.shape[1])},
+    }
+
+
 def fp64_oracle_error(fw, x, ktraj, smaps) -> float:
     """Forward NUFFT vs FP64 NUDFT on a 1-coil slice and a sample subset."""
     from oracle import nudft
@@ -301,6 +362,8 @@ def main() -> int:
     ap.add_argument("--stage-iters", type=int, default=20)
     ap.add_argument("--form-iters", type=int, default=5, help="iterations of the batched / gradient forms")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
+    ap.add_argument("--workload", default="sense2d", choices=["sense2d", "kooshball3d"],
+                    help="sense2d: the headline SENSE pair (default); kooshball3d: the 3-D deterministic case")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -314,6 +377,9 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload == "kooshball3d":
+        print(json.dumps(bench_kooshball(dev, max(1, args.steps // 10), args.warmup)), flush=True)
+        return 0
     gen = torch.Generator().manual_seed(0)
     ktraj_cpu = radial_golden(NSPOKES, NREAD)
     M = ktraj_cpu.shape[1]

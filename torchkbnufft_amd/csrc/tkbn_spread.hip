@@ -1265,8 +1265,8 @@ __global__ void __launch_bounds__(256)
 // host side
 // ---------------------------------------------------------------------------
 // Tile shapes tried in order (largest first): the first whose LDS staging holds
-// min(rows, 8) coil rows at once is used, so a sample's weights are evaluated once
-// per tile for small coil counts; large row counts take the first shape.
+// min(rows, 3) coil rows at once is used.  Larger tiles keep more samples (lanes)
+// busy per group; the row floor keeps 3-D tiles from holding a single row.
 static const int* tile_shape(int D, bool fwd, int64_t nrows, int64_t numpoints_max, bool f32) {
   static const int spread2[2][3] = {{32, 32, 1}, {16, 16, 1}};
   static const int spread3[3][3] = {{16, 16, 8}, {8, 8, 8}, {8, 8, 4}};
@@ -1282,7 +1282,7 @@ static const int* tile_shape(int D, bool fwd, int64_t nrows, int64_t numpoints_m
     cand = fwd ? fwd3 : spread3;
     n = fwd ? 1 : 3;
   }
-  const int64_t want = nrows < 8 ? (nrows > 0 ? nrows : 1) : 8;
+  const int64_t want = nrows < 3 ? (nrows > 0 ? nrows : 1) : 3;
   for (int c = 0; c < n; ++c) {
     int64_t cells = 1;
     for (int i = 0; i < D; ++i) cells *= fwd ? cand[c][i] + numpoints_max - 1 : cand[c][i];
