@@ -341,6 +341,18 @@ def bench_kooshball(dev, steps: int, warmup: int) -> dict:
     }
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (profiles/traffic_sense2d_radial_640.*, calibrated on a plain copy
+    kernel); None when that profile is absent or does not cover the kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic_sense2d_radial_640.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def fp64_oracle_error(fw, x, ktraj, smaps) -> float:
     """Forward NUFFT vs FP64 NUDFT on a 1-coil slice and a sample subset."""
     from oracle import nudft
@@ -362,8 +374,9 @@ def main() -> int:
     ap.add_argument("--stage-iters", type=int, default=20)
     ap.add_argument("--form-iters", type=int, default=5, help="iterations of the batched / gradient forms")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
-    ap.add_argument("--workload", default="sense2d", choices=["sense2d", "kooshball3d"],
-                    help="sense2d: the headline SENSE pair (default); kooshball3d: the 3-D deterministic case")
+    ap.add_argument("--workload", default="sense2d", choices=["sense2d", "kooshball3d", "copy512"],
+                    help="sense2d: the headline SENSE pair (default); kooshball3d: the 3-D deterministic case; "
+                    "copy512: a plain 512 MiB read + 512 MiB write kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -379,6 +392,14 @@ def main() -> int:
     torch.cuda.set_device(dev)
     if args.workload == "kooshball3d":
         print(json.dumps(bench_kooshball(dev, max(1, args.steps // 10), args.warmup)), flush=True)
+        return 0
+    if args.workload == "copy512":
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)  # 512 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"workload": "copy512", "read_bytes": src.numel() * 4, "write_bytes": dst.numel() * 4}))
         return 0
     gen = torch.Generator().manual_seed(0)
     ktraj_cpu = radial_golden(NSPOKES, NREAD)
@@ -443,7 +464,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom),
         },
         "pair": {
             "algorithmic_bytes": pair_bytes,
