@@ -378,12 +378,17 @@ __global__ void __launch_bounds__(1024)
   const int K = pl.K;
   float2* tw = reinterpret_cast<float2*>(smem);
   float2* buf = tw + K;
-  build_twiddles(tw, K);
+  // Groups are dealt to the 8 XCDs round-robin; each XCD takes a contiguous range of
+  // (row, column block) items instead, so neighbouring column blocks, which share
+  // cache lines, meet in one L2.  The grid is a multiple of 8: a bijection.
+  const int64_t per_xcd = gridDim.x >> 3;
+  const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
   const int64_t chunks = (inner + ncol - 1) / ncol;
-  const int64_t row = blockIdx.x / chunks;
-  const int64_t col0 = (blockIdx.x - row * chunks) * ncol;
-  const int nc = (int)(col0 + ncol <= inner ? ncol : inner - col0);
+  const int64_t row = item / chunks;
+  const int64_t col0 = (item - row * chunks) * ncol;
   if (row >= rows) return;
+  build_twiddles(tw, K);
+  const int nc = (int)(col0 + ncol <= inner ? ncol : inner - col0);
   const float2* s = src + row * (int64_t)n_in * inner + col0;
   batched_copy(
       K * ncol,
@@ -502,7 +507,7 @@ static int launch_mid(bool inv, const FftPlan1D& pl, int64_t rows, int n_in, int
                       const float2* src, float2* dst, hipStream_t s) {
   const int ncol = mid_cols(pl);
   const int64_t chunks = ceil_div(inner, ncol);
-  const int64_t groups = rows * chunks;
+  const int64_t groups = ceil_div(rows * chunks, 8) * 8;  // a multiple of 8: see fft_mid_kernel
   TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_INVALID, "too many FFT columns");
   const size_t lds = (size_t)(pl.K + (int64_t)pl.K * ncol) * sizeof(float2);
   const FastDiv fcol = make_fastdiv((uint32_t)ncol);

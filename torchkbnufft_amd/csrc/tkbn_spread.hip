@@ -521,11 +521,12 @@ __global__ void __launch_bounds__(256)
 //   registers, so every cell has exactly one writer and no atomics are needed.
 //   The plan stores, for every (sample, tile) entry, a record with the sample's tap
 //   weights (phase folded into dimension 0, both conjugated) and the footprint's first
-//   row r0 and column c0 relative to the tile, and orders each tile's entries by c0.
-//   The kernel stages 64 entries at a time (records, and the entries' data with
-//   lanes = entries so consecutive trajectory samples load coalesced), then walks the
-//   c0 groups in order: inside a group the column offsets are compile-time constants,
-//   so every accumulator index is fixed and the taps are plain FMAs.
+//   row r0 and column c0 relative to the tile, and keeps each tile's entries close to
+//   trajectory order.  The kernel stages 64 entries at a time (records, and the
+//   entries' data with lanes = entries, so the samples of one spoke load coalesced),
+//   then walks the staged entries c0 value by c0 value (one ballot each): inside a
+//   group the column offsets are compile-time constants, so every accumulator index
+//   is fixed and the taps are plain FMAs.
 //   Footprints are contiguous inside a tile when every grid extent is at least
 //   tile + J - 1 cells (a wrapped footprint then never meets one tile twice);
 //   smaller grids use the sample-lane kernel.
@@ -648,8 +649,13 @@ __global__ void __launch_bounds__(256)
   rec.pad[0] = rec.pad[1] = 0;
   int* plan = ws;
   CoilRec* recs = reinterpret_cast<CoilRec*>(reinterpret_cast<char*>(ws) + tg.o_rec_bytes);
+  // Positions come from the tile's cursor (its first key), not the (tile, c0) key's:
+  // a tile's entries then keep trajectory order as far as the atomics preserve it, so
+  // the samples of one spoke crossing the tile sit together and the kernel's loads of
+  // their data share cache lines.
   coil_for_each_entry(tg, tl, off, n, live, [&](int key, int a, int c, bool valid) {
-    const int pos = wave_aggregated_inc(plan + tg.o_cursor, key, valid);
+    const int tkey = valid ? key - (off[1][c] - kRegMinC0) : 0;
+    const int pos = wave_aggregated_inc(plan + tg.o_cursor, tkey, valid);
     if (valid) {
       plan[tg.o_perm + pos] = m;
       rec.r0 = off[0][a];
@@ -756,9 +762,9 @@ __device__ __forceinline__ void reg_add_row(float2 (&acc)[kRegT1], const float2 
 }
 
 // one entry with footprint column C0 whose rows meet the wave's rows; the weights are
-// read from the plan record at a wave-uniform address (scalar loads)
+// read from the staged record in LDS at a wave-uniform address (broadcast reads)
 template <int C0>
-__device__ __forceinline__ void reg_entry(float2 (&acc)[kRegRowsPerWave][kRegT1], const CoilRec* __restrict__ rec,
+__device__ __forceinline__ void reg_entry(float2 (&acc)[kRegRowsPerWave][kRegT1], const CoilRec* rec,
                                           const float2 v, int ja, int J0) {
   float2 w1[6];
 #pragma unroll
@@ -780,6 +786,7 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
     spread_coil_regs_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ kdata,
                             const int* __restrict__ ws, float2* __restrict__ grid) {
   __shared__ __attribute__((aligned(16))) float2 vals[kRegBatch * kRegPitch];
+  __shared__ __attribute__((aligned(16))) CoilRec brec[kRegBatch];  // the batch's records
   // wave-uniform in a scalar register, so the row tests below are scalar branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -811,17 +818,16 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
 #pragma unroll
     for (int c = 0; c < kRegT1; ++c) acc[h][c] = make_float2(0.f, 0.f);
 
-  // Software pipeline: while a batch is processed, the next batch's data and entry
-  // offsets are loaded into registers, and the trajectory indices one batch further.
-  int2 prc = make_int2(0, 0);  // (r0, c0) of entry `lane` of the next batch
+  // Software pipeline: while a batch is processed, the next batch's data and records
+  // are loaded into registers (vector loads: the records' first touch of memory is
+  // never a wait inside the entry loop), and the trajectory indices one batch further.
+  constexpr int kRecF4 = (int)(sizeof(CoilRec) / sizeof(float4));  // 7
+  static_assert(kRegBatch * kRecF4 <= kRegThreads, "one record float4 per thread");
+  float4 prec = make_float4(0.f, 0.f, 0.f, 0.f);
   float2 pv[8];
   auto prefetch = [&](int b, int mb) {
     const int n = min(kRegBatch, i1 - b);
-    prc = make_int2(1000, 1000);
-    if (lane < n) {
-      const CoilRec* r = recs + b + lane;
-      prc = make_int2(r->r0, r->c0);
-    }
+    if ((int)threadIdx.x < n * kRecF4) prec = reinterpret_cast<const float4*>(recs + b)[threadIdx.x];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int row = wave * 8 + q;
@@ -839,8 +845,11 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
   for (int base = i0; base < i1; base += kRegBatch) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) vals[lane * kRegPitch + wave * 8 + q] = pv[q];
-    const int r0l = prc.x, c0l = prc.y;  // entry `lane` of this batch (1000: none)
+    const int nb = min(kRegBatch, i1 - base);
+    if ((int)threadIdx.x < nb * kRecF4) reinterpret_cast<float4*>(brec)[threadIdx.x] = prec;
     __syncthreads();
+    // entry `lane` of this batch (1000: none)
+    const int r0l = lane < nb ? brec[lane].r0 : 1000, c0l = lane < nb ? brec[lane].c0 : 1000;
     const int b1 = base + kRegBatch;
     if (b1 < i1) {
       prefetch(b1, mnext);
@@ -850,8 +859,7 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
     // entries whose footprint rows meet this wave's rows
     const int jal = myrow - r0l;
     const unsigned long long hits = __ballot(jal + kRegRowsPerWave > 0 && jal < J0);
-    const CoilRec* brec = recs + base;
-    // entries are ordered by c0: walk each c0 value's hits with fixed column offsets
+    // walk each c0 value's hits with fixed column offsets
 #define TKBN_REG_GROUP(C0)                                                       \
   {                                                                              \
     unsigned long long mk = __ballot(c0l == (C0)) & hits;                        \
@@ -908,18 +916,22 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
 //   The plan bins every sample once, into the tile of its base cell (the
 //   footprint's first cell), keeping trajectory order inside a tile, and stores a
 //   per-entry record: phase * w0[j], w1[j] and the base cell relative to the tile.
-//   One group (8 waves) per (tile, split, chunk of 64 coil rows) stages the tile and
-//   its J-1 halo (wrapped) in LDS with the coil row fastest, so a tap read by the 64
-//   lanes (= coil rows) of a wave is one conflict-free LDS access.  Each wave takes
-//   one entry at a time; its weights are read at a wave-uniform address.
+//   One group (8 waves) per (8 x 16 tile, split, chunk of 32 coil rows) stages the
+//   tile and its J-1 halo (13 x 21 cells, wrapped) in LDS with the coil row fastest.
+//   Each half-wave takes one entry (lanes = the chunk's 32 rows), so a tap read by a
+//   wave is two runs of 32 consecutive LDS words and conflict-free.  Larger tiles
+//   than 64-row chunks allow stage 2.1x the tile's cells instead of 3.7x and give
+//   each staging round 4x the samples to hide its latency behind.
 // ---------------------------------------------------------------------------
-constexpr int kGatT0 = 4, kGatT1 = 8;       // tile edges (the region fits twice per CU)
+constexpr int kGatT0 = 8, kGatT1 = 16;      // tile edges
 constexpr int kGatR0 = kGatT0 + 5;          // staged region edges (J <= 6)
 constexpr int kGatR1 = kGatT1 + 5;
-constexpr int kGatCells = kGatR0 * kGatR1;
-constexpr int kGatPitch = 65;               // float2 per staged cell: 64 rows + 1
+constexpr int kGatCells = kGatR0 * kGatR1;  // 273
+constexpr int kGatRows = 32;                // coil rows per work item
+constexpr int kGatPitch = kGatRows + 1;     // float2 per staged cell (odd: staging stores spread over banks)
 constexpr int kGatThreads = 512;
 constexpr int kGatTarget = 256;             // entries per work item
+static_assert(kGatCells * kGatPitch * 8 * 2 <= 160 * 1024, "two groups per CU");
 
 struct GatRec {
   float2 w0[6];  // phase * w0[j] along dimension 0
@@ -1040,12 +1052,16 @@ __device__ __forceinline__ int64_t xcd_work_item(int64_t b) {
 
 __global__ void __launch_bounds__(kGatThreads)
     gather_coils_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ grid,
-                        const int* ws, float2* kdata) {  // no restrict: keeps the record loads vector loads
+                        const int* ws, float2* kdata, int nchunk) {  // no restrict: keeps the record loads vector loads
   __shared__ __attribute__((aligned(16))) float2 region[kGatCells * kGatPitch];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  const int half = lane >> 5, row = lane & 31;
   const int* plan = ws;
-  const int64_t wi = xcd_work_item(blockIdx.x);
+  // the chunks of one work item are consecutive, so they meet in one XCD's L2
+  const int64_t item = xcd_work_item(blockIdx.x);
+  const int64_t wi = item / nchunk;
+  const int chunk = (int)(item - wi * nchunk);
   if (wi >= plan[tg.o_nwork]) return;
   const int* wstart = plan + tg.o_wstart;
   const int t = plan[tg.o_work + wi];
@@ -1055,52 +1071,56 @@ __global__ void __launch_bounds__(kGatThreads)
   const int i1 = min(i0 + tg.target, V);
   if (i0 >= i1) return;  // no sample is based in this tile
   const int64_t nrows = g.B * g.C;
-  const int64_t rbase = (int64_t)blockIdx.y * 64;
-  const int ncc = (int)((rbase + 64 < nrows ? rbase + 64 : nrows) - rbase);
+  const int64_t rbase = (int64_t)chunk * kGatRows;
+  const int ncc = (int)((rbase + kGatRows < nrows ? rbase + kGatRows : nrows) - rbase);
   const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kGatT0;
   const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kGatT1;
 
-  // stage region[cell][row] = grid[row][(o + l) mod K], cell = l0 * kGatR1 + l1.  A
-  // thread owns one cell and every third row: all its loads are issued before any
-  // LDS store, so the staging costs one memory latency.
-  constexpr int kGroups = kGatThreads / kGatCells;          // 3 row groups
-  constexpr int kPer = (64 + kGroups - 1) / kGroups;        // 22 rows per thread
-  const int cell = threadIdx.x % kGatCells, cg = threadIdx.x / kGatCells;
-  int64_t off = 0;
-  {
-    const int l0 = cell / kGatR1, l1 = cell - l0 * kGatR1;
-    int k0 = o0 + l0, k1 = o1 + l1;
-    while (k0 >= g.K[0]) k0 -= g.K[0];
-    while (k1 >= g.K[1]) k1 -= g.K[1];
-    off = (int64_t)k0 * g.stride[0] + k1;
-  }
-  float2 v[kPer];
+  // stage region[cell][r] = grid[rbase + r][(o + l) mod K], cell = l0 * kGatR1 + l1:
+  // consecutive threads take consecutive cells of one row (runs of 21 cells are
+  // contiguous in memory); every load is issued before any LDS store.
+  constexpr int kTotal = kGatCells * kGatRows;
+  constexpr int kLoads = (kTotal + kGatThreads - 1) / kGatThreads;  // 18
+  float2 v[kLoads];
 #pragma unroll
-  for (int q = 0; q < kPer; ++q) {
-    const int c = cg + q * kGroups;
+  for (int q = 0; q < kLoads; ++q) {
+    const int idx = threadIdx.x + q * kGatThreads;
+    const int r = idx / kGatCells, cell = idx - r * kGatCells;
     v[q] = make_float2(0.f, 0.f);
-    if (cg < kGroups && c < ncc) v[q] = grid[(rbase + c) * g.G + off];
+    if (idx < kTotal && r < ncc) {
+      const int l0 = cell / kGatR1, l1 = cell - l0 * kGatR1;
+      int k0 = o0 + l0, k1 = o1 + l1;
+      while (k0 >= g.K[0]) k0 -= g.K[0];
+      while (k1 >= g.K[1]) k1 -= g.K[1];
+      v[q] = grid[(rbase + r) * g.G + (int64_t)k0 * g.stride[0] + k1];
+    }
   }
 #pragma unroll
-  for (int q = 0; q < kPer; ++q) {
-    const int c = cg + q * kGroups;
-    if (cg < kGroups && c < ncc) region[cell * kGatPitch + c] = v[q];
+  for (int q = 0; q < kLoads; ++q) {
+    const int idx = threadIdx.x + q * kGatThreads;
+    const int r = idx / kGatCells, cell = idx - r * kGatCells;
+    if (idx < kTotal) region[cell * kGatPitch + r] = v[q];
   }
   __syncthreads();
 
-  // Records are read with vector loads (uniform address), one entry ahead of use: the
-  // LDS reads then never wait on them (scalar loads would share the LDS wait counter).
+  // Records are read with vector loads, one entry ahead of use (scalar loads would
+  // share the LDS wait counter).  Half-wave h of wave w takes entries
+  // i0 + 2w + h, stepping by 16.
   const float4* recs = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) +
                        (int64_t)plan[tg.o_start + t] * 7;
   const int J0 = g.J[0], J1 = g.J[1];
-  constexpr int kStep = kGatThreads / 64;
+  constexpr int kStep = 2 * (kGatThreads / 64);
   float4 cur[7], nxt[7];
-  int e = i0 + wave;
-  if (e < i1) {
+  {
+    const int e = i0 + 2 * wave + half;
+    if (e < i1) {
 #pragma unroll
-    for (int q = 0; q < 7; ++q) cur[q] = recs[(int64_t)e * 7 + q];
+      for (int q = 0; q < 7; ++q) cur[q] = recs[(int64_t)e * 7 + q];
+    }
   }
-  for (; e < i1; e += kStep) {
+  for (int eb = i0 + 2 * wave; eb < i1; eb += kStep) {
+    const int e = eb + half;
+    const bool act = e < i1;
     if (e + kStep < i1) {
 #pragma unroll
       for (int q = 0; q < 7; ++q) nxt[q] = recs[(int64_t)(e + kStep) * 7 + q];
@@ -1114,34 +1134,38 @@ __global__ void __launch_bounds__(kGatThreads)
       w1[2 * q] = make_float2(cur[3 + q].x, cur[3 + q].y);
       w1[2 * q + 1] = make_float2(cur[3 + q].z, cur[3 + q].w);
     }
-    const int l0 = __builtin_amdgcn_readfirstlane(__float_as_int(cur[6].x));
-    const int l1 = __builtin_amdgcn_readfirstlane(__float_as_int(cur[6].y));
-    const int m = __builtin_amdgcn_readfirstlane(__float_as_int(cur[6].z));
+    const int l0 = __float_as_int(cur[6].x), l1 = __float_as_int(cur[6].y), m = __float_as_int(cur[6].z);
     float2 acc = make_float2(0.f, 0.f);
+    if (act) {
+      const float2* L0 = region + (l0 * kGatR1 + l1) * kGatPitch + row;
 #pragma unroll
-    for (int j0 = 0; j0 < 6; ++j0) {
-      if (j0 < J0) {
-        const float2* L = region + ((l0 + j0) * kGatR1 + l1) * kGatPitch + lane;
-        float2 p = make_float2(0.f, 0.f);
+      for (int j0 = 0; j0 < 6; ++j0) {
+        if (j0 < J0) {
+          const float2* L = L0 + j0 * (kGatR1 * kGatPitch);
+          // four independent FMA chains (real and imaginary parts of both products)
+          float2 p = make_float2(0.f, 0.f), q = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int j1 = 0; j1 < 6; ++j1) {
-          if (j1 < J1) {
-            const float2 w = w1[j1];
-            const float2 v = L[j1 * kGatPitch];
-            p.x = fmaf(w.x, v.x, p.x);
-            p.x = fmaf(-w.y, v.y, p.x);
-            p.y = fmaf(w.x, v.y, p.y);
-            p.y = fmaf(w.y, v.x, p.y);
+          for (int j1 = 0; j1 < 6; ++j1) {
+            if (j1 < J1) {
+              const float2 w = w1[j1];
+              const float2 x = L[j1 * kGatPitch];
+              p.x = fmaf(w.x, x.x, p.x);
+              q.x = fmaf(-w.y, x.y, q.x);
+              p.y = fmaf(w.x, x.y, p.y);
+              q.y = fmaf(w.y, x.x, q.y);
+            }
           }
+          p.x += q.x;
+          p.y += q.y;
+          const float2 w = w0[j0];
+          acc.x = fmaf(w.x, p.x, acc.x);
+          acc.x = fmaf(-w.y, p.y, acc.x);
+          acc.y = fmaf(w.x, p.y, acc.y);
+          acc.y = fmaf(w.y, p.x, acc.y);
         }
-        const float2 w = w0[j0];
-        acc.x = fmaf(w.x, p.x, acc.x);
-        acc.x = fmaf(-w.y, p.y, acc.x);
-        acc.y = fmaf(w.x, p.y, acc.y);
-        acc.y = fmaf(w.y, p.x, acc.y);
       }
+      if (row < ncc) kdata[(rbase + row) * g.M + m] = acc;
     }
-    if (lane < ncc) kdata[(rbase + lane) * g.M + m] = acc;
 #pragma unroll
     for (int q = 0; q < 7; ++q) cur[q] = nxt[q];
   }
@@ -1671,7 +1695,7 @@ static TileGeom make_gather_geom(const tkbn_interp_desc* d) {
   tg.bstride[2] = 0;
   tg.NB = tg.nb[0] * tg.nb[1];
   tg.own_only = 1;
-  tg.CC = 64;
+  tg.CC = kGatRows;
   tg.target = kGatTarget;
   const int64_t M = d->klength;
   tg.maxent = M;
@@ -1719,11 +1743,12 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
     st = plan_cache_commit(d, kPlanGatherCoils, tg.T, omega, ws, s);
     if (st != TKBN_OK) return st;
   }
-  const unsigned ychunks = (unsigned)ceil_div(g.B * g.C, 64);
-  const int64_t groups = ceil_div(tg.maxwork, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
+  const int nchunk = (int)ceil_div(g.B * g.C, kGatRows);
+  const int64_t groups = ceil_div(tg.maxwork * nchunk, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
+  TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_UNSUPPORTED, "too many coil rows x work items for one launch");
   profile_before(s);
-  gather_coils_kernel<<<dim3((unsigned)groups, ychunks, 1), dim3(kGatThreads), 0, s>>>(
-      g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata));
+  gather_coils_kernel<<<dim3((unsigned)groups), dim3(kGatThreads), 0, s>>>(
+      g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata), nchunk);
   profile_after(s);
   TKBN_LAUNCH_CHECK("gather_coils_kernel");
   return TKBN_OK;
