@@ -280,6 +280,55 @@ def bench_grad(dev, iters: int) -> dict:
     }
 
 
+def bench_dcomp(dev, ktraj, iters: int, with_cpu: bool) -> dict:
+    """Density compensation (10 spread + gather iterations, one row) on the radial
+    trajectory; the CPU oracle's time for the same call when the baseline is on."""
+    import torchkbnufft_amd as tkbn
+
+    om = ktraj.to(dev)
+    fn = lambda: tkbn.calc_density_compensation_function(ktraj=om, im_size=(N, N))  # noqa: E731
+    out = {
+        "workload": f"calc_density_compensation_function, {N}x{N} -> {K}x{K}, {om.shape[-1]} radial samples, "
+        "10 iterations (interp_adjoint then interp, one row)",
+        "gpu_ms": timed(fn, iters),
+    }
+    if with_cpu:
+        from oracle import table_path as tp
+
+        plan = tp.Plan((N, N), dtype=torch.complex64)
+        t0 = time.perf_counter()
+        tp.density_compensation(plan, ktraj.cpu())
+        out["cpu_oracle_ms"] = (time.perf_counter() - t0) * 1e3
+        out["cpu_threads"] = torch.get_num_threads()
+    return out
+
+
+def bench_toeplitz(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
+    """Toeplitz normal operator on the SENSE workload: one ToepNufft call replaces the
+    forward + adjoint pair; algorithmic bytes = read image + smaps, the grid once
+    through the FFT filter (written, read, kernel read), write image."""
+    import torchkbnufft_amd as tkbn
+
+    t0 = time.perf_counter()
+    kernel = tkbn.calc_toeplitz_kernel(ktraj, (N, N))
+    torch.cuda.synchronize()
+    kernel_ms = (time.perf_counter() - t0) * 1e3
+    toep = tkbn.ToepNufft()
+    ms = timed(lambda: toep(x, kernel, smaps=smaps), iters)
+    G2, P = (2 * N) ** 2, N * N
+    alg = B * P * 8 + B * C * P * 8 + 2 * (2 * B * C * G2 * 8) + G2 * 8 + B * C * P * 8 + B * P * 8
+    return {
+        "workload": f"ToepNufft on the same batch ({B} x {C} coils, {N}x{N}), kernel {2 * N}x{2 * N} from the "
+        "radial trajectory; A^H A x without interpolation",
+        "kernel_build_ms": kernel_ms,
+        "toep_ms": ms,
+        "nufft_pair_ms": pair_ms,
+        "speedup_vs_pair": pair_ms / ms,
+        "algorithmic_bytes": alg,
+        "hbm_frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+
+
 def bench_kooshball(dev, steps: int, warmup: int) -> dict:
     """3-D radial kooshball: 160^3 -> 320^3, 8 coils with Gaussian-smoothed maps, 3200
     spokes uniform on the sphere x 512 samples (1 638 400), deterministic spread;
@@ -479,6 +528,8 @@ def main() -> int:
     if not args.no_forms:
         out["batched"] = bench_batched(dev, args.form_iters)
         out["grad"] = bench_grad(dev, args.form_iters)
+        out["dcomp"] = bench_dcomp(dev, ktraj, args.form_iters, not args.no_cpu_baseline)
+        out["toeplitz"] = bench_toeplitz(dev, x, ktraj, smaps, max(args.form_iters, 10), ms)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, ktraj, smaps, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

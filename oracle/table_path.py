@@ -245,6 +245,137 @@ def kb_table_interp_adjoint(plan: Plan, data, omega, grid_size=None):
     )
 
 
+def density_compensation(plan: Plan, omega, num_iterations: int = 10):
+    """Pipe's iteration (``dcomp.py:97-117``): ``w <- w / |G G^H w|`` from ``w = 1``.
+
+    ``omega`` is ``(d, M)`` or ``(B, d, M)``; returns ``(B, 1, M)`` in the plan's
+    complex dtype.  ``G^H`` / ``G`` are this module's adjoint / forward
+    interpolation on the plan's grid.
+    """
+    if omega.ndim == 3 and omega.shape[0] == 1:
+        omega = omega[0]
+    batch = omega.shape[0] if omega.ndim == 3 else 1
+    w = torch.ones((batch, 1, omega.shape[-1]), dtype=plan.tables[0].dtype)
+    for _ in range(num_iterations):
+        back = kb_table_interp(plan, kb_table_interp_adjoint(plan, w, omega), omega)
+        w = w / torch.abs(back)
+    return w
+
+
+# ----------------------------------------------------------------------------
+# Toeplitz embedding (toep.py:11-300, fft.py:121-173, kbnufft.py:413-547)
+# ----------------------------------------------------------------------------
+def _adjoint_flip_concat(dim, omega, weights, adjoint):
+    """toep.py:185-236: adjoints over the sign flips of dims >= ``dim``, concatenated."""
+    ax = dim + 2
+    flip = torch.ones(omega.shape[0], dtype=omega.dtype).unsqueeze(-1)
+    flip[dim] = -1
+    if dim < omega.shape[0] - 1:
+        k1 = _adjoint_flip_concat(dim + 1, omega, weights, adjoint)
+        k2 = _adjoint_flip_concat(dim + 1, flip * omega, weights, adjoint)
+    else:
+        k1 = adjoint(weights, omega)
+        k2 = adjoint(weights, flip * omega)
+    zshape = list(k1.shape)
+    zshape[ax] = 1
+    k2 = k2.narrow(ax, 1, k2.shape[ax] - 1)
+    return torch.cat((k1, torch.zeros(zshape, dtype=k1.dtype), k2.flip(ax)), ax)
+
+
+def _reverse_index(x, d):
+    return x.index_select(d, torch.remainder(-1 * torch.arange(x.shape[d]), x.shape[d]))
+
+
+def _reflect_conj_concat(kernel, dim):
+    """toep.py:239-274"""
+    tmp = kernel.conj()
+    for d in range(dim, kernel.ndim):
+        tmp = _reverse_index(tmp, d)
+    zshape = list(kernel.shape)
+    zshape[dim] = 1
+    tmp = torch.cat((torch.zeros(zshape, dtype=kernel.dtype), tmp.narrow(dim, 1, tmp.shape[dim] - 1)), dim)
+    return torch.cat((kernel, tmp), dim)
+
+
+def _hermitify(kernel, dim):
+    """toep.py:277-300"""
+    rev = kernel
+    for d in range(dim, kernel.ndim):
+        rev = _reverse_index(rev, d)
+    return (kernel + rev.conj()) / 2
+
+
+def toeplitz_kernel(
+    omega, im_size, weights=None, norm=None, grid_size=None, numpoints=6, dtype=torch.complex64, adjoint=None
+):
+    """calc_toeplitz_kernel (``toep.py:11-182``) for a ``(d, M)`` or ``(B, d, M)`` omega.
+
+    ``adjoint(w, omega)`` replaces the table NUFFT adjoint (``n_shift = 0``) when
+    given, e.g. by an exact NUDFT adjoint for the exactness check.
+    """
+    if omega.ndim == 3:
+        return torch.stack(
+            [
+                toeplitz_kernel(
+                    om, im_size, None if weights is None else weights[i], norm, grid_size, numpoints, dtype, adjoint
+                )
+                for i, om in enumerate(omega)
+            ]
+        )
+    d = omega.shape[0]
+    plan = Plan(im_size, grid_size, numpoints=numpoints, n_shift=[0] * d, dtype=dtype)
+    if adjoint is None:
+        adjoint = lambda w_, om_: kb_table_nufft_adjoint(plan, w_, om_, norm=norm)  # noqa: E731
+    if weights is None:
+        w = torch.ones(omega.shape[-1], dtype=dtype).unsqueeze(0).unsqueeze(0)
+    else:
+        w = weights.to(dtype).unsqueeze(0)
+    if d > 1:
+        kernel = _adjoint_flip_concat(1, omega, w, adjoint)
+    else:
+        kernel = adjoint(w, omega)
+    kernel = _hermitify(_reflect_conj_concat(kernel, 2), 2)
+    im = torch.tensor(plan.im_size, dtype=torch.float64)
+    if norm == "ortho":
+        scale = torch.sqrt(torch.prod(2 * im) / torch.prod(torch.tensor(plan.grid_size, dtype=torch.float64)))
+    else:
+        scale = 1 / torch.prod(2 * im)
+    dims = list(range(-d, 0))
+    return torch.fft.fftn(kernel, dim=dims, norm="ortho" if norm == "ortho" else None)[0, 0] * scale.item()
+
+
+def fft_filter(image, kernel, norm="ortho"):
+    """fft.py:121-173: crop(iFFT(kernel * FFT(pad_end(image))))."""
+    d = image.ndim - 2
+    im = image.shape[2:]
+    gs = kernel.shape[-d:]
+    pad = []
+    for g, n in zip(reversed(list(gs)), reversed(list(im))):
+        pad += [0, int(g) - int(n)]
+    dims = list(range(-d, 0))
+    x = torch.fft.fftn(F.pad(image, pad), dim=dims, norm="ortho" if norm == "ortho" else None) * kernel
+    x = torch.fft.ifftn(x, dim=dims, norm="ortho" if norm == "ortho" else "forward")
+    for ax, n in enumerate(im):
+        x = torch.narrow(x, 2 + ax, 0, int(n))
+    return x
+
+
+def toep_nufft(image, kernel, smaps=None, norm=None):
+    """ToepNufft.forward (``kbnufft.py:491-547``) on complex tensors."""
+    d = image.ndim - 2
+    if kernel.ndim > d and kernel.shape[0] == 1:
+        kernel = kernel[0]
+    if smaps is None:
+        return fft_filter(image, kernel, norm)
+    out = []
+    for b in range(image.shape[0]):
+        kb = kernel[b] if kernel.ndim > d else kernel
+        sm = smaps[b if smaps.shape[0] > 1 else 0].unsqueeze(0)
+        x = fft_filter(image[b].unsqueeze(0) * sm, kb, norm)
+        out.append(torch.sum(x * sm.conj(), dim=1, keepdim=True)[0])
+    return torch.stack(out)
+
+
 def inner_product(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
     """``<a, b> = sum(conj(b) * a)`` (``_math.py:138-166``)."""
     return torch.sum(b.conj() * a)
@@ -258,6 +389,10 @@ def rel_l2(a, b) -> float:
 
 __all__: List[str] = [
     "Plan",
+    "density_compensation",
+    "toeplitz_kernel",
+    "fft_filter",
+    "toep_nufft",
     "table_interp",
     "table_interp_adjoint",
     "fft_and_scale",

@@ -148,6 +148,31 @@ def test_cpu_tensors_are_refused_not_emulated():
         adj(torch.randn(1, 1, 5, dtype=torch.complex64), torch.zeros(2, 5))
 
 
+def test_dcomp_and_toeplitz_validation_matches_reference_errors():
+    with pytest.raises(ValueError, match="ktraj must have 2 or 3 dimensions"):
+        tkbn.calc_density_compensation_function(torch.zeros(5), (8, 8))
+    with pytest.raises(ValueError, match="Unrecognized k-space shape"):
+        tkbn.calc_toeplitz_kernel(torch.zeros(5), (8, 8))
+    with pytest.raises(ValueError, match="Unrecognized weights dimension"):
+        tkbn.calc_toeplitz_kernel(torch.zeros(2, 5), (8, 8), weights=torch.ones(5))
+    with pytest.raises(ValueError, match="same batch size"):
+        tkbn.calc_toeplitz_kernel(torch.zeros(3, 2, 5), (8, 8), weights=torch.ones(2, 5))
+    toep = tkbn.ToepNufft()
+    image = torch.randn(1, 1, 8, 8, dtype=torch.complex64)
+    with pytest.raises(TypeError, match="same dtype"):
+        toep(image, torch.randn(16, 16, dtype=torch.complex128))
+    with pytest.raises(TypeError, match="smaps"):
+        toep(image, torch.randn(16, 16, dtype=torch.complex64), smaps=torch.randn(1, 2, 8, 8, dtype=torch.complex128))
+    with pytest.raises(ValueError, match="last dimension must be size 2"):
+        toep(torch.randn(1, 1, 8, 8, 3), torch.randn(16, 16, 3))
+    with pytest.raises(ValueError, match="same batch size as image"):
+        toep(image, torch.randn(3, 16, 16, dtype=torch.complex64))
+    with pytest.raises(ValueError, match="Only option for norm is 'ortho'"):
+        tkbn.functional.fft_filter(image, torch.randn(16, 16, dtype=torch.complex64), norm="forward")
+    with pytest.raises(RuntimeError, match="GPU"):
+        toep(image, torch.randn(16, 16, dtype=torch.complex64))
+
+
 def test_math_helpers():
     a = torch.randn(3, 4, dtype=torch.complex128)
     b = torch.randn(3, 4, dtype=torch.complex128)

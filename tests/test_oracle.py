@@ -177,3 +177,102 @@ def test_kb_derivative_matches_fd():
     # central difference of the table itself, away from the support edges (jumps)
     cd = (v[3:-2] - v[1:-4]) * 1024 / 2
     assert np.abs(cd - t[2:-3]).max() / np.abs(t).max() < 1e-4
+
+
+def _dense_interp_matrix(plan, om, grid):
+    """Columns of the forward interpolation applied to unit grids: (M, prod(grid))."""
+    g = int(np.prod(grid))
+    cols = []
+    for col in range(g):
+        e = torch.zeros((1, 1, g), dtype=plan.tables[0].dtype)
+        e[0, 0, col] = 1
+        cols.append(tp.kb_table_interp(plan, e.reshape((1, 1) + tuple(grid)), om)[0, 0])
+    return torch.stack(cols, dim=1)
+
+
+def test_density_compensation_matches_dense_iteration():
+    """Pipe's iteration on the dense interpolation matrix A (w <- w / |A A^H w|)."""
+    torch.manual_seed(5)
+    im, grid = (5, 6), (10, 12)
+    plan = tp.Plan(im, grid, numpoints=4, dtype=torch.complex128)
+    om = create_ktraj(2, 30)
+    A = _dense_interp_matrix(plan, om, grid).numpy()
+    w = np.ones(30, dtype=np.complex128)
+    for _ in range(7):
+        w = w / np.abs(A @ (A.conj().T @ w))
+    out = tp.density_compensation(plan, om, num_iterations=7)
+    assert out.shape == (1, 1, 30)
+    np.testing.assert_allclose(out[0, 0].numpy(), w, rtol=1e-12, atol=0)
+
+
+def test_density_compensation_uniform_grid_is_flat_fixed_point():
+    """Samples on every oversampled grid point: the kernel matrix is circulant, so
+    the weights are equal and |A A^H w| = 1 (the iteration's fixed point).  The
+    trajectory's float rounding puts a few samples a table step off the grid point,
+    hence the loose flatness bound."""
+    im, grid = (6, 5), (12, 10)
+    plan = tp.Plan(im, grid, numpoints=4, dtype=torch.complex128)
+    k0, k1 = torch.meshgrid(torch.arange(grid[0]), torch.arange(grid[1]), indexing="ij")
+    om = torch.stack([2 * np.pi * k0.reshape(-1) / grid[0], 2 * np.pi * k1.reshape(-1) / grid[1]]).double()
+    om = torch.remainder(om + np.pi, 2 * np.pi) - np.pi
+    w = tp.density_compensation(plan, om)
+    assert float(w.real.max() / w.real.min()) - 1 < 1e-5
+    back = tp.kb_table_interp(plan, tp.kb_table_interp_adjoint(plan, w, om), om)
+    assert torch.allclose(back.abs(), torch.ones_like(back.real), rtol=0, atol=1e-7)
+
+
+def test_density_compensation_batched_is_loop():
+    torch.manual_seed(6)
+    plan = tp.Plan((7, 8), numpoints=6, dtype=torch.complex128)
+    om = torch.rand((3, 2, 50), dtype=torch.float64) * 2 * np.pi - np.pi
+    loop = torch.cat([tp.density_compensation(plan, o) for o in om])
+    assert torch.allclose(tp.density_compensation(plan, om), loop, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("im_size", [(9,), (6, 5), (4, 5, 3)])
+@pytest.mark.parametrize("norm", [None, "ortho"])
+def test_toeplitz_assembly_is_exact_with_nudft_psf(im_size, norm):
+    """With the exact NUDFT adjoint as the point-spread function, the assembled kernel
+    reproduces A^H W A of the NUDFT to rounding: the flips, the Hermitian half, the
+    zero middle row and the scaling are all checked."""
+    torch.manual_seed(7)
+    d = len(im_size)
+    om = create_ktraj(d, 40)
+    w = torch.rand(1, 40, dtype=torch.float64)
+    sc = 1.0 if norm is None else 1.0 / np.sqrt(np.prod([2 * n for n in im_size]))
+
+    def exact_adjoint(wt, o):
+        return torch.from_numpy(nudft.nudft_adjoint(wt.numpy(), o.numpy(), im_size, [0] * d)) * sc
+
+    kernel = tp.toeplitz_kernel(om, im_size, weights=w, norm=norm, dtype=torch.complex128, adjoint=exact_adjoint)
+    x = torch.complex(torch.randn((1, 1) + im_size, dtype=torch.float64), torch.randn((1, 1) + im_size, dtype=torch.float64))
+    y = nudft.nudft(x.numpy(), om.numpy(), [0] * d) * w.numpy()
+    ref = nudft.nudft_adjoint(y, om.numpy(), im_size, [0] * d)
+    if norm == "ortho":
+        ref = ref / np.prod([2 * n for n in im_size])
+    out = tp.toep_nufft(x, kernel, norm=norm)
+    assert rel_l2(out, torch.from_numpy(ref)) < 1e-10
+
+
+@pytest.mark.parametrize(
+    "shape, grid_size, M, norm, tol",
+    [
+        ([1, 3, 19], [57], 25, "ortho", 1e-2),
+        ([1, 4, 32, 16], [64, 24], 83, None, 1e-2),
+        ([2, 4, 32, 16], None, 83, "ortho", 1e-4),
+        ([1, 3, 7, 6, 5], [14, 12, 10], 60, None, 1e-4),
+    ],
+)
+def test_toeplitz_oracle_approximates_normal_operator(shape, grid_size, M, norm, tol):
+    """The reference's own accuracy criterion (tests/test_toep.py:9-55) on the oracle."""
+    torch.manual_seed(123)
+    im_size = tuple(shape[2:])
+    B, C = shape[0], shape[1]
+    x = create_input_plus_noise([B, 1] + list(im_size), True)
+    smaps = create_input_plus_noise(shape, True)
+    om = create_ktraj(len(im_size), M)
+    plan = tp.Plan(im_size, grid_size, dtype=torch.complex128)
+    kernel = tp.toeplitz_kernel(om, im_size, norm=norm, grid_size=grid_size, dtype=torch.complex128)
+    fbn = tp.kb_table_nufft_adjoint(plan, tp.kb_table_nufft(plan, x, om, smaps, norm), om, smaps, norm)
+    fbt = tp.toep_nufft(x, kernel, smaps, norm)
+    assert rel_l2(fbt, fbn) < tol

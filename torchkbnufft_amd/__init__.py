@@ -12,8 +12,10 @@ from . import functional, modules
 from .batched import BatchedKbNufft
 from ._math import absolute, complex_mult, complex_sign, conj_complex_mult, imag_exp, inner_product
 from ._nufft import utils as nufft_utils
+from ._nufft.dcomp import calc_density_compensation_function
+from ._nufft.toep import calc_toeplitz_kernel
 from ._nufft.interp import clear_plan_cache, deterministic_enabled, set_deterministic
-from .modules import KbInterp, KbInterpAdjoint, KbNufft, KbNufftAdjoint
+from .modules import KbInterp, KbInterpAdjoint, KbNufft, KbNufftAdjoint, ToepNufft
 
 __version__ = "0.1.0"
 
@@ -22,7 +24,10 @@ __all__ = [
     "KbInterpAdjoint",
     "KbNufft",
     "KbNufftAdjoint",
+    "ToepNufft",
     "absolute",
+    "calc_density_compensation_function",
+    "calc_toeplitz_kernel",
     "complex_mult",
     "complex_sign",
     "conj_complex_mult",

@@ -234,3 +234,58 @@ def fft_grid_adj(grid: Tensor, smaps: Optional[Tensor], scaling_coef: Tensor, im
         )
     _lib.check(st, "tkbn_fft_grid_adj")
     return out
+
+
+# ---------------------------------------------------------------------------
+# FFT filtering on an oversampled grid (Toeplitz embedding)
+# ---------------------------------------------------------------------------
+_ONES = {}
+
+
+def _ones_scaling(im, dtype, device) -> Tensor:
+    key = (tuple(im), dtype, device)
+    t = _ONES.get(key)
+    if t is None:
+        t = torch.ones(tuple(im), dtype=dtype, device=device)
+        _ONES[key] = t
+    return t
+
+
+def toeplitz_filter(image: Tensor, smaps: Optional[Tensor], kernel: Tensor, norm: Optional[str]) -> Tensor:
+    """crop(iFFT(kernel * FFT(pad_end(image [* smaps])))) [summed against conj(smaps)].
+
+    The grid size is the kernel's trailing extents.  Where the fused FFT covers the
+    geometry (complex64, 2/3/5-smooth sizes) the pad + FFT (+ SENSE expansion) and
+    the iFFT + crop (+ coil combine) are the fused HIP pipelines with a unit
+    scaling, and only the kernel product runs as a separate element-wise pass.
+    """
+    from .._autograd.fft import GridAdjoint, GridForward
+
+    if norm is not None and norm != "ortho":
+        raise ValueError("Only option for norm is 'ortho'.")
+    require_gpu(image, smaps, kernel)
+    d = image.ndim - 2
+    im = tuple(int(v) for v in image.shape[2:])
+    gs = tuple(int(v) for v in kernel.shape[-d:])
+    scale = _norm_scale(norm, gs)
+    if fused_supported(image.dtype, im, gs) and all(g >= n for g, n in zip(gs, im)):
+        ones = _ones_scaling(im, image.dtype, image.device)
+        grid = GridForward.apply(image, smaps, ones, im, gs, scale)
+        return GridAdjoint.apply(grid * kernel, smaps, ones, im, scale)
+    x = image * smaps if smaps is not None else image
+    pad = []
+    for g, n in zip(reversed(gs), reversed(im)):
+        pad += [0, g - n]
+    dims = tuple(range(-d, 0))
+    grid = torch.fft.fftn(torch.nn.functional.pad(x, pad), dim=dims, norm="ortho" if norm == "ortho" else None)
+    x = torch.fft.ifftn(grid * kernel, dim=dims, norm="ortho" if norm == "ortho" else "forward")
+    for ax, n in enumerate(im):
+        x = x.narrow(2 + ax, 0, n)
+    if smaps is not None:
+        x = torch.sum(x * smaps.conj(), dim=1, keepdim=True)
+    return x
+
+
+def fft_filter(image: Tensor, kernel: Tensor, norm: Optional[str] = "ortho") -> Tensor:
+    """``iFFT(kernel * FFT(image))`` on an oversampled grid (``fft.py:121-173``)."""
+    return toeplitz_filter(image, None, kernel, norm)

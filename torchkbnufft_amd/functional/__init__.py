@@ -1,10 +1,11 @@
 """Functional API (``torchkbnufft/functional/__init__.py``), table mode."""
-from .._nufft.fft import fft_and_scale, ifft_and_scale
+from .._nufft.fft import fft_and_scale, fft_filter, ifft_and_scale
 from .interp import kb_table_interp, kb_table_interp_adjoint
 from .nufft import kb_table_nufft, kb_table_nufft_adjoint
 
 __all__ = [
     "fft_and_scale",
+    "fft_filter",
     "ifft_and_scale",
     "kb_table_interp",
     "kb_table_interp_adjoint",

@@ -1,4 +1,5 @@
 from .kbinterp import KbInterp, KbInterpAdjoint
 from .kbnufft import KbNufft, KbNufftAdjoint
+from .toepnufft import ToepNufft
 
-__all__ = ["KbInterp", "KbInterpAdjoint", "KbNufft", "KbNufftAdjoint"]
+__all__ = ["KbInterp", "KbInterpAdjoint", "KbNufft", "KbNufftAdjoint", "ToepNufft"]
