@@ -20,6 +20,11 @@ GEOMS = [
     ((20, 18, 10), (40, 36, 20), 1, 3),
     ((64, 96), (128, 192), 1, 2),
     ((7, 5), (10, 6), 3, 2),
+    # lengths with compiled passes (256, 320, 384, 512, 640, 768) in every pass kind
+    ((192, 128), (384, 256), 1, 2),
+    ((160, 300), (320, 768), 1, 2),
+    ((200,), (512,), 2, 2),
+    ((60, 10, 100), (320, 20, 640), 1, 2),
 ]
 
 

@@ -137,19 +137,22 @@ __device__ void build_twiddles(float2* tw, int K) {
 // Group-wide copy of `total` elements: store(i, load(i)) for i < total, with kCopyU
 // loads in flight per thread before their stores (a loop of single loads would
 // wait one memory latency per element).
+// Every FFT kernel runs with kFftThreads threads (the launches and __launch_bounds__).
+constexpr int kFftThreads = 1024;  // 16 waves per group: the stash stays small (<=128 VGPRs)
+constexpr int kFftLdsBudget = 60 * 1024;  // twiddles + lines per group, two groups per CU
 constexpr int kCopyU = 8;
 template <typename LoadF, typename StoreF>
 __device__ __forceinline__ void batched_copy(int total, LoadF load, StoreF store) {
-  for (int base = threadIdx.x; base < total; base += kCopyU * blockDim.x) {
+  for (int base = threadIdx.x; base < total; base += kCopyU * kFftThreads) {
     float2 v[kCopyU];
 #pragma unroll
     for (int u = 0; u < kCopyU; ++u) {
-      const int i = base + u * blockDim.x;
+      const int i = base + u * kFftThreads;
       v[u] = i < total ? load(i) : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int u = 0; u < kCopyU; ++u) {
-      const int i = base + u * blockDim.x;
+      const int i = base + u * kFftThreads;
       if (i < total) store(i, v[u]);
     }
   }
@@ -163,7 +166,6 @@ __device__ __forceinline__ void batched_copy(int total, LoadF load, StoreF store
 // (at most kMaxBfly, guaranteed by the host's choice of ntr), the group syncs, then
 // every thread writes them back.
 constexpr int kMaxBfly = 2;
-constexpr int kFftThreads = 1024;  // 16 waves per group: the stash stays small (<=128 VGPRs)
 
 template <bool INV, int R, bool T_MAJOR>
 __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int K, int ntr, int st, int sk,
@@ -175,7 +177,7 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int
   float2 vals[kMaxBfly][R];
 #pragma unroll
   for (int p = 0; p < kMaxBfly; ++p) {
-    const int b = threadIdx.x + p * blockDim.x;
+    const int b = threadIdx.x + p * kFftThreads;
     if (b < total) {
       const int t = T_MAJOR ? (int)fdiv(b, nbd) : (int)fmodd(b, ntrd);
       const int j = T_MAJOR ? b - t * nb : (int)fdiv(b, ntrd);
@@ -200,7 +202,7 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < kMaxBfly; ++p) {
-    const int b = threadIdx.x + p * blockDim.x;
+    const int b = threadIdx.x + p * kFftThreads;
     if (b < total) {
       const int t = T_MAJOR ? (int)fdiv(b, nbd) : (int)fmodd(b, ntrd);
       const int j = T_MAJOR ? b - t * nb : (int)fdiv(b, ntrd);
@@ -226,6 +228,106 @@ __device__ void lds_fft(float2* buf, const float2* tw, const FftPlan1D& pl, int 
       case 3: stockham_pass<INV, 3, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, pl.nb[p], pl.ns[p], ntrd); break;
       default: stockham_pass<INV, 2, T_MAJOR>(buf, tw, pl.K, ntr, st, sk, pl.nb[p], pl.ns[p], ntrd); break;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the same passes with the length known at compile time: every index division,
+// stride and twiddle step folds into constants (multiply-shift or shifts).  The
+// radix order is the one factorize() chooses, so results equal the runtime path's.
+// ---------------------------------------------------------------------------
+struct RadixList {
+  int n;  // 0: K has a prime factor other than 2, 3, 5
+  int r[kMaxPasses];
+};
+
+constexpr RadixList radix_list(int K) {
+  RadixList L{0, {}};
+  int r = K, n5 = 0, n3 = 0, e = 0;
+  while (r % 5 == 0) { r /= 5; ++n5; }
+  while (r % 3 == 0) { r /= 3; ++n3; }
+  while (r % 2 == 0) { r /= 2; ++e; }
+  if (r != 1) return L;
+  int n8 = 0, n4 = 0, n2 = 0;
+  if (e == 1) {
+    n2 = 1;
+  } else {
+    n8 = e / 3;
+    if (e % 3 == 1) { n8 -= 1; n4 = 2; }
+    if (e % 3 == 2) { n4 = 1; }
+  }
+  const int counts[5] = {n8, n5, n4, n3, n2};
+  const int radix[5] = {8, 5, 4, 3, 2};
+  for (int i = 0; i < 5; ++i)
+    for (int c = 0; c < counts[i]; ++c) L.r[L.n++] = radix[i];
+  return L;
+}
+
+constexpr int radices_before(int K, int p) {
+  const RadixList L = radix_list(K);
+  int ns = 1;
+  for (int q = 0; q < p; ++q) ns *= L.r[q];
+  return ns;
+}
+
+template <bool INV, int R, bool T_MAJOR, int K, int NS, int NTRC>
+__device__ __forceinline__ void stockham_pass_c(float2* buf, const float2* tw, int ntr) {
+  constexpr int nb = K / R;
+  constexpr int step = nb / NS;
+  constexpr int st = T_MAJOR ? K : 1;
+  constexpr int sk = T_MAJOR ? 1 : NTRC;
+  const int total = ntr * nb;
+  float2 vals[kMaxBfly][R];
+#pragma unroll
+  for (int p = 0; p < kMaxBfly; ++p) {
+    const int b = threadIdx.x + p * kFftThreads;
+    if (b < total) {
+      const int t = T_MAJOR ? b / nb : b % NTRC;
+      const int j = T_MAJOR ? b - t * nb : b / NTRC;
+      float2 x[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) x[i] = buf[t * st + (j + i * nb) * sk];
+      if constexpr (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int i = 1; i < R; ++i) {
+          float2 w = tw[i * k * step];
+          if (INV) w.y = -w.y;
+          const float2 a = x[i];
+          x[i] = make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+        }
+      }
+      dft<INV, R>(x);
+#pragma unroll
+      for (int i = 0; i < R; ++i) vals[p][i] = x[i];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < kMaxBfly; ++p) {
+    const int b = threadIdx.x + p * kFftThreads;
+    if (b < total) {
+      const int t = T_MAJOR ? b / nb : b % NTRC;
+      const int j = T_MAJOR ? b - t * nb : b / NTRC;
+      const int jq = j / NS;
+      const int k = j - jq * NS;
+      const int base = jq * NS * R + k;
+#pragma unroll
+      for (int i = 0; i < R; ++i) buf[t * st + (base + i * NS) * sk] = vals[p][i];
+    }
+  }
+  __syncthreads();
+}
+
+// ntr transforms of the compile-time length K; NTRC: transforms per group in the
+// column layout (ntr == NTRC there), unused in the row layout
+template <bool INV, bool T_MAJOR, int K, int NTRC, int P = 0>
+__device__ __forceinline__ void lds_fft_c(float2* buf, const float2* tw, int ntr) {
+  constexpr RadixList L = radix_list(K);
+  static_assert(L.n > 0, "length must be 2/3/5-smooth");
+  if constexpr (P < L.n) {
+    stockham_pass_c<INV, L.r[P], T_MAJOR, K, radices_before(K, P), NTRC>(buf, tw, ntr);
+    lds_fft_c<INV, T_MAJOR, K, NTRC, P + 1>(buf, tw, ntr);
   }
 }
 
@@ -258,13 +360,15 @@ struct Extents {
 // (N_0..N_{d-2}, K_{d-1}).  In MODE 1 lines run over (b, n_0..n_{d-2}) of the image and
 // the group loops over coils.
 // ---------------------------------------------------------------------------
-template <int MODE>
+// KC > 0: the last grid extent is KC (compile-time passes and index arithmetic).
+template <int MODE, int KC>
 __global__ void __launch_bounds__(1024)
     fft_last_kernel(const FftGeom g, const float2* __restrict__ src, const float2* __restrict__ smaps,
                     const float2* __restrict__ scaling, float2* __restrict__ dst, int lines_per_group) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int D = g.ndim;
-  const int K = g.K[D - 1], N = g.N[D - 1];
+  const int K = KC > 0 ? KC : g.K[D - 1], N = g.N[D - 1];
+  auto divK = [&](int x) { return KC > 0 ? x / KC : (int)fdiv(x, g.fK); };
   float2* tw = reinterpret_cast<float2*>(smem);
   float2* buf = tw + K;
   build_twiddles(tw, K);
@@ -282,7 +386,7 @@ __global__ void __launch_bounds__(1024)
     batched_copy(
         nl * K,
         [&](int idx) {
-          const int l = (int)fdiv(idx, g.fK), k = idx - l * K;
+          const int l = divK(idx), k = idx - l * K;
           const int64_t line = line0 + l;  // < 2^31 (checked on the host)
           const int64_t bc = fdiv((uint32_t)line, g.fQ), q = line - bc * Q;
           const int64_t b = fdiv((uint32_t)bc, g.fC), c = bc - b * g.C;
@@ -299,7 +403,10 @@ __global__ void __launch_bounds__(1024)
         },
         [&](int idx, float2 v) { buf[idx] = v; });
     __syncthreads();
-    lds_fft<false, true>(buf, tw, g.plan[D - 1], nl, K, 1, g.fK);
+    if constexpr (KC > 0)
+      lds_fft_c<false, true, KC, 1>(buf, tw, nl);
+    else
+      lds_fft<false, true>(buf, tw, g.plan[D - 1], nl, K, 1, g.fK);
     batched_copy(
         nl * K, [&](int idx) { return buf[idx]; }, [&](int idx, float2 v) { dst[line0 * K + idx] = v; });
   } else {
@@ -315,12 +422,15 @@ __global__ void __launch_bounds__(1024)
       batched_copy(
           nc * K,
           [&](int idx) {
-            const int l = (int)fdiv(idx, g.fK), k = idx - l * K;
+            const int l = divK(idx), k = idx - l * K;
             return src[((b * g.C + c0 + l) * Q + q) * K + k];
           },
           [&](int idx, float2 v) { buf[idx] = v; });
       __syncthreads();
-      lds_fft<true, true>(buf, tw, g.plan[D - 1], nc, K, 1, g.fK);
+      if constexpr (KC > 0)
+        lds_fft_c<true, true, KC, 1>(buf, tw, nc);
+      else
+        lds_fft<true, true>(buf, tw, g.plan[D - 1], nc, K, 1, g.fK);
       if (smaps) {
         if (n < N) {
           const float2 s = scaling[q * N + n];
@@ -370,12 +480,28 @@ __global__ void __launch_bounds__(1024)
 // the product of the extents after d.  One group: NC adjacent inner columns of one row.
 // Forward: n_in = N[d] zero-padded to K; adjoint (INV): n_out = N[d] (crop).
 // ---------------------------------------------------------------------------
-template <bool INV>
+// columns per group in a pass of length K (the host's mid_cols for the same K)
+constexpr int mid_cols_c(int K) {
+  const RadixList L = radix_list(K);
+  int mr = 8;
+  for (int p = 0; p < L.n; ++p) mr = L.r[p] < mr ? L.r[p] : mr;
+  const int maxtr = kMaxBfly * kFftThreads * mr / K;
+  int ncol = 16;
+  while (ncol > 1 && ((ncol + 1) * K * 8 > kFftLdsBudget || ncol > maxtr)) ncol /= 2;
+  return ncol;
+}
+
+// KC > 0: the pass length is KC and the group takes mid_cols_c(KC) columns
+// (compile-time passes and index arithmetic); ncol / fcol then equal that width.
+template <bool INV, int KC>
 __global__ void __launch_bounds__(1024)
-    fft_mid_kernel(const FftPlan1D pl, int64_t rows, int n_in, int n_out, int64_t inner, int ncol,
+    fft_mid_kernel(const FftPlan1D pl, int64_t rows, int n_in, int n_out, int64_t inner, int ncol_rt,
                    const FastDiv fcol, const float2* __restrict__ src, float2* __restrict__ dst) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  const int K = pl.K;
+  constexpr int NCC = KC > 0 ? mid_cols_c(KC) : 1;
+  const int K = KC > 0 ? KC : pl.K;
+  const int ncol = KC > 0 ? NCC : ncol_rt;
+  auto divc = [&](int x) { return KC > 0 ? x / NCC : (int)fdiv(x, fcol); };
   float2* tw = reinterpret_cast<float2*>(smem);
   float2* buf = tw + K;
   // Groups are dealt to the 8 XCDs round-robin; each XCD takes a contiguous range of
@@ -393,17 +519,20 @@ __global__ void __launch_bounds__(1024)
   batched_copy(
       K * ncol,
       [&](int idx) {
-        const int k = (int)fdiv(idx, fcol), t = idx - k * ncol;
+        const int k = divc(idx), t = idx - k * ncol;
         return (k < n_in && t < nc) ? s[(int64_t)k * inner + t] : make_float2(0.f, 0.f);
       },
       [&](int idx, float2 v) { buf[idx] = v; });
   __syncthreads();
-  lds_fft<INV, false>(buf, tw, pl, ncol, 1, ncol, fcol);
+  if constexpr (KC > 0)
+    lds_fft_c<INV, false, KC, NCC>(buf, tw, NCC);
+  else
+    lds_fft<INV, false>(buf, tw, pl, ncol, 1, ncol, fcol);
   float2* o = dst + row * (int64_t)n_out * inner + col0;
   batched_copy(
       n_out * ncol, [&](int idx) { return buf[idx]; },
       [&](int idx, float2 v) {
-        const int k = (int)fdiv(idx, fcol), t = idx - k * ncol;
+        const int k = divc(idx), t = idx - k * ncol;
         if (t < nc) o[(int64_t)k * inner + t] = v;
       });
 }
@@ -457,7 +586,6 @@ static int max_transforms(const FftPlan1D& pl) {
   return (int)((int64_t)kMaxBfly * kFftThreads * min_radix(pl) / pl.K);
 }
 
-constexpr int kFftLdsBudget = 60 * 1024;
 constexpr int kMaxK = 2048;
 
 static int check_fft(int32_t ndim, const int64_t* im, const int64_t* gs, FftGeom* g) {
@@ -503,6 +631,24 @@ static int mid_cols(const FftPlan1D& pl) {
   return ncol;
 }
 
+// Grid extents whose passes are compiled for their length (others use the runtime plan).
+#define TKBN_FFT_COMPILED_SIZES(X) X(256) X(320) X(384) X(512) X(640) X(768)
+
+template <int MODE>
+static void launch_last(int K, unsigned groups, size_t lds, hipStream_t s, const FftGeom& g, const float2* src,
+                        const float2* smaps, const float2* scaling, float2* dst, int lpg) {
+  switch (K) {
+#define TKBN_LAST(KV)                                                                                  \
+  case KV:                                                                                             \
+    fft_last_kernel<MODE, KV><<<dim3(groups), dim3(kFftThreads), lds, s>>>(g, src, smaps, scaling, dst, lpg); \
+    return;
+    TKBN_FFT_COMPILED_SIZES(TKBN_LAST)
+#undef TKBN_LAST
+    default:
+      fft_last_kernel<MODE, 0><<<dim3(groups), dim3(kFftThreads), lds, s>>>(g, src, smaps, scaling, dst, lpg);
+  }
+}
+
 static int launch_mid(bool inv, const FftPlan1D& pl, int64_t rows, int n_in, int n_out, int64_t inner,
                       const float2* src, float2* dst, hipStream_t s) {
   const int ncol = mid_cols(pl);
@@ -511,12 +657,28 @@ static int launch_mid(bool inv, const FftPlan1D& pl, int64_t rows, int n_in, int
   TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_INVALID, "too many FFT columns");
   const size_t lds = (size_t)(pl.K + (int64_t)pl.K * ncol) * sizeof(float2);
   const FastDiv fcol = make_fastdiv((uint32_t)ncol);
+  const dim3 grid((unsigned)groups), block(kFftThreads);
+  switch (pl.K) {
+#define TKBN_MID(KV)                                                                                       \
+  case KV:                                                                                                 \
+    if (ncol == mid_cols_c(KV)) {                                                                          \
+      if (inv)                                                                                             \
+        fft_mid_kernel<true, KV><<<grid, block, lds, s>>>(pl, rows, n_in, n_out, inner, ncol, fcol, src, dst);  \
+      else                                                                                                 \
+        fft_mid_kernel<false, KV><<<grid, block, lds, s>>>(pl, rows, n_in, n_out, inner, ncol, fcol, src, dst); \
+      TKBN_LAUNCH_CHECK("fft_mid_kernel");                                                                 \
+      return TKBN_OK;                                                                                      \
+    }                                                                                                      \
+    break;
+    TKBN_FFT_COMPILED_SIZES(TKBN_MID)
+#undef TKBN_MID
+    default:
+      break;
+  }
   if (inv)
-    fft_mid_kernel<true><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(pl, rows, n_in, n_out, inner, ncol,
-                                                                                 fcol, src, dst);
+    fft_mid_kernel<true, 0><<<grid, block, lds, s>>>(pl, rows, n_in, n_out, inner, ncol, fcol, src, dst);
   else
-    fft_mid_kernel<false><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(pl, rows, n_in, n_out, inner, ncol,
-                                                                                  fcol, src, dst);
+    fft_mid_kernel<false, 0><<<grid, block, lds, s>>>(pl, rows, n_in, n_out, inner, ncol, fcol, src, dst);
   TKBN_LAUNCH_CHECK("fft_mid_kernel");
   return TKBN_OK;
 }
@@ -604,9 +766,8 @@ int tkbn_fft_grid_fwd(int32_t dtype, int32_t ndim, const int64_t* im_size, const
     const int64_t groups = ceil_div(nlines, lpg);
     float2* out = D == 1 ? static_cast<float2*>(grid) : bufa;
     const size_t lds = (size_t)(K + (int64_t)lpg * K) * sizeof(float2);
-    fft_last_kernel<0><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(
-        g, static_cast<const float2*>(image), static_cast<const float2*>(smaps),
-        static_cast<const float2*>(scaling_coef), out, lpg);
+    launch_last<0>(K, (unsigned)groups, lds, s, g, static_cast<const float2*>(image),
+                   static_cast<const float2*>(smaps), static_cast<const float2*>(scaling_coef), out, lpg);
     TKBN_LAUNCH_CHECK("fft_last_kernel<fwd>");
   }
   // remaining dimensions, innermost first
@@ -679,17 +840,16 @@ int tkbn_fft_grid_adj(int32_t dtype, int32_t ndim, const int64_t* im_size, const
     const size_t lds = (size_t)(K + (int64_t)cc * K) * sizeof(float2);
     if (smaps) {
       const int64_t groups = g.B * Q;
-      fft_last_kernel<1><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(
-          g, cur, static_cast<const float2*>(smaps), static_cast<const float2*>(scaling_coef),
-          static_cast<float2*>(image), cc);
+      launch_last<1>(K, (unsigned)groups, lds, s, g, cur, static_cast<const float2*>(smaps),
+                     static_cast<const float2*>(scaling_coef), static_cast<float2*>(image), cc);
     } else {
       // per-coil output: treat each (b, c) as its own batch entry with one coil
       FftGeom g1 = g;
       g1.B = g.B * g.C;
       g1.C = 1;
       const int64_t groups = g1.B * Q;
-      fft_last_kernel<1><<<dim3((unsigned)groups), dim3(kFftThreads), lds, s>>>(
-          g1, cur, nullptr, static_cast<const float2*>(scaling_coef), static_cast<float2*>(image), 1);
+      launch_last<1>(K, (unsigned)groups, lds, s, g1, cur, nullptr, static_cast<const float2*>(scaling_coef),
+                     static_cast<float2*>(image), 1);
     }
     TKBN_LAUNCH_CHECK("fft_last_kernel<adj>");
   }
