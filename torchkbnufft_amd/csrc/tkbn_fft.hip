@@ -141,17 +141,17 @@ __device__ void build_twiddles(float2* tw, int K) {
 constexpr int kFftThreads = 1024;  // 16 waves per group: the stash stays small (<=128 VGPRs)
 constexpr int kFftLdsBudget = 60 * 1024;  // twiddles + lines per group, two groups per CU
 constexpr int kCopyU = 8;
-template <typename LoadF, typename StoreF>
+template <int U = kCopyU, typename LoadF, typename StoreF>
 __device__ __forceinline__ void batched_copy(int total, LoadF load, StoreF store) {
-  for (int base = threadIdx.x; base < total; base += kCopyU * kFftThreads) {
-    float2 v[kCopyU];
+  for (int base = threadIdx.x; base < total; base += U * kFftThreads) {
+    float2 v[U];
 #pragma unroll
-    for (int u = 0; u < kCopyU; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int i = base + u * kFftThreads;
       v[u] = i < total ? load(i) : make_float2(0.f, 0.f);
     }
 #pragma unroll
-    for (int u = 0; u < kCopyU; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int i = base + u * kFftThreads;
       if (i < total) store(i, v[u]);
     }
@@ -271,7 +271,7 @@ constexpr int radices_before(int K, int p) {
 }
 
 template <bool INV, int R, bool T_MAJOR, int K, int NS, int NTRC>
-__device__ __forceinline__ void stockham_pass_c(float2* buf, const float2* tw, int ntr) {
+__device__ __forceinline__ void stockham_pass_c(float2* buf, const float2* tw, int ntr, int tid) {
   constexpr int nb = K / R;
   constexpr int step = nb / NS;
   constexpr int st = T_MAJOR ? K : 1;
@@ -280,7 +280,7 @@ __device__ __forceinline__ void stockham_pass_c(float2* buf, const float2* tw, i
   float2 vals[kMaxBfly][R];
 #pragma unroll
   for (int p = 0; p < kMaxBfly; ++p) {
-    const int b = threadIdx.x + p * kFftThreads;
+    const int b = tid + p * kFftThreads;
     if (b < total) {
       const int t = T_MAJOR ? b / nb : b % NTRC;
       const int j = T_MAJOR ? b - t * nb : b / NTRC;
@@ -305,7 +305,7 @@ __device__ __forceinline__ void stockham_pass_c(float2* buf, const float2* tw, i
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < kMaxBfly; ++p) {
-    const int b = threadIdx.x + p * kFftThreads;
+    const int b = tid + p * kFftThreads;
     if (b < total) {
       const int t = T_MAJOR ? b / nb : b % NTRC;
       const int j = T_MAJOR ? b - t * nb : b / NTRC;
@@ -320,14 +320,16 @@ __device__ __forceinline__ void stockham_pass_c(float2* buf, const float2* tw, i
 }
 
 // ntr transforms of the compile-time length K; NTRC: transforms per group in the
-// column layout (ntr == NTRC there), unused in the row layout
+// column layout (ntr == NTRC there), unused in the row layout.  tid is threadIdx.x;
+// a caller inside a loop passes it through an empty asm so that the passes' index
+// arithmetic is not hoisted out of the loop (it would hold ~40 more registers).
 template <bool INV, bool T_MAJOR, int K, int NTRC, int P = 0>
-__device__ __forceinline__ void lds_fft_c(float2* buf, const float2* tw, int ntr) {
+__device__ __forceinline__ void lds_fft_c(float2* buf, const float2* tw, int ntr, int tid) {
   constexpr RadixList L = radix_list(K);
   static_assert(L.n > 0, "length must be 2/3/5-smooth");
   if constexpr (P < L.n) {
-    stockham_pass_c<INV, L.r[P], T_MAJOR, K, radices_before(K, P), NTRC>(buf, tw, ntr);
-    lds_fft_c<INV, T_MAJOR, K, NTRC, P + 1>(buf, tw, ntr);
+    stockham_pass_c<INV, L.r[P], T_MAJOR, K, radices_before(K, P), NTRC>(buf, tw, ntr, tid);
+    lds_fft_c<INV, T_MAJOR, K, NTRC, P + 1>(buf, tw, ntr, tid);
   }
 }
 
@@ -355,14 +357,15 @@ struct Extents {
 // ---------------------------------------------------------------------------
 // last-dimension pass (contiguous lines).  One group: LINES lines of one (b, c).
 //   MODE 0: forward first pass, input = image * smaps * scaling * scale (n < N), zero pad
-//   MODE 1: adjoint last pass, output = crop * conj(scaling) * scale [* conj(smaps), summed]
+//   MODE 1: adjoint last pass, output = sum_c crop * conj(scaling) * scale * conj(smaps_c)
+//   MODE 2: adjoint last pass without smaps, output = crop * conj(scaling) * scale per coil
 // In MODE 0 lines run over (b, c, n_0..n_{d-2}) with n < N; output extents
 // (N_0..N_{d-2}, K_{d-1}).  In MODE 1 lines run over (b, n_0..n_{d-2}) of the image and
 // the group loops over coils.
 // ---------------------------------------------------------------------------
 // KC > 0: the last grid extent is KC (compile-time passes and index arithmetic).
 template <int MODE, int KC>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))  // two groups per CU
     fft_last_kernel(const FftGeom g, const float2* __restrict__ src, const float2* __restrict__ smaps,
                     const float2* __restrict__ scaling, float2* __restrict__ dst, int lines_per_group) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -404,7 +407,7 @@ __global__ void __launch_bounds__(1024)
         [&](int idx, float2 v) { buf[idx] = v; });
     __syncthreads();
     if constexpr (KC > 0)
-      lds_fft_c<false, true, KC, 1>(buf, tw, nl);
+      lds_fft_c<false, true, KC, 1>(buf, tw, nl, threadIdx.x);
     else
       lds_fft<false, true>(buf, tw, g.plan[D - 1], nl, K, 1, g.fK);
     batched_copy(
@@ -419,6 +422,8 @@ __global__ void __launch_bounds__(1024)
     float2 acc = make_float2(0.f, 0.f);
     for (int64_t c0 = 0; c0 < g.C; c0 += cc) {
       const int nc = (int)((c0 + cc <= g.C) ? cc : g.C - c0);
+      int tid = threadIdx.x;
+      asm volatile("" : "+v"(tid));  // keeps the passes' index arithmetic inside the loop
       batched_copy(
           nc * K,
           [&](int idx) {
@@ -428,20 +433,21 @@ __global__ void __launch_bounds__(1024)
           [&](int idx, float2 v) { buf[idx] = v; });
       __syncthreads();
       if constexpr (KC > 0)
-        lds_fft_c<true, true, KC, 1>(buf, tw, nc);
+        lds_fft_c<true, true, KC, 1>(buf, tw, nc, tid);
       else
         lds_fft<true, true>(buf, tw, g.plan[D - 1], nc, K, 1, g.fK);
-      if (smaps) {
+      if constexpr (MODE == 1) {
         if (n < N) {
           const float2 s = scaling[q * N + n];
-          // smaps of up to kCopyU coils in flight before they are used
-          for (int l0 = 0; l0 < nc; l0 += kCopyU) {
-            float2 sm[kCopyU];
+          // smaps of up to 4 coils in flight before they are used
+          constexpr int kSm = 4;
+          for (int l0 = 0; l0 < nc; l0 += kSm) {
+            float2 sm[kSm];
 #pragma unroll
-            for (int u = 0; u < kCopyU; ++u)
+            for (int u = 0; u < kSm; ++u)
               sm[u] = l0 + u < nc ? smaps[((b * g.C + c0 + l0 + u) * Q + q) * N + n] : make_float2(0.f, 0.f);
 #pragma unroll
-            for (int u = 0; u < kCopyU; ++u) {
+            for (int u = 0; u < kSm; ++u) {
               if (l0 + u < nc) {
                 float2 v = cmul_conj(buf[(l0 + u) * K + n], s);
                 v.x *= g.scale;
@@ -454,7 +460,7 @@ __global__ void __launch_bounds__(1024)
           }
         }
       } else {
-        batched_copy(
+        batched_copy<4>(  // 4 in flight: two operands per element
             nc * N,
             [&](int idx) {
               const int l = (int)fdiv(idx, g.fN), nn = idx - l * N;
@@ -470,7 +476,7 @@ __global__ void __launch_bounds__(1024)
       }
       __syncthreads();
     }
-    if (smaps && n < N) dst[(b * Q + q) * N + n] = acc;
+    if (MODE == 1 && n < N) dst[(b * Q + q) * N + n] = acc;
   }
 }
 
@@ -525,7 +531,7 @@ __global__ void __launch_bounds__(1024)
       [&](int idx, float2 v) { buf[idx] = v; });
   __syncthreads();
   if constexpr (KC > 0)
-    lds_fft_c<INV, false, KC, NCC>(buf, tw, NCC);
+    lds_fft_c<INV, false, KC, NCC>(buf, tw, NCC, threadIdx.x);
   else
     lds_fft<INV, false>(buf, tw, pl, ncol, 1, ncol, fcol);
   float2* o = dst + row * (int64_t)n_out * inner + col0;
@@ -848,7 +854,7 @@ int tkbn_fft_grid_adj(int32_t dtype, int32_t ndim, const int64_t* im_size, const
       g1.B = g.B * g.C;
       g1.C = 1;
       const int64_t groups = g1.B * Q;
-      launch_last<1>(K, (unsigned)groups, lds, s, g1, cur, nullptr, static_cast<const float2*>(scaling_coef),
+      launch_last<2>(K, (unsigned)groups, lds, s, g1, cur, nullptr, static_cast<const float2*>(scaling_coef),
                      static_cast<float2*>(image), 1);
     }
     TKBN_LAUNCH_CHECK("fft_last_kernel<adj>");
