@@ -766,14 +766,21 @@ __device__ __forceinline__ void reg_add_row(float2 (&acc)[kRegT1], const float2 
 template <int C0>
 __device__ __forceinline__ void reg_entry(float2 (&acc)[kRegRowsPerWave][kRegT1], const CoilRec* rec,
                                           const float2 v, int ja, int J0) {
-  float2 w1[6];
+  // every operand is read before the first use (one LDS wait per entry): the row
+  // weights at clamped indices, used only for the rows inside the footprint
+  float2 w1[6], w0r[kRegRowsPerWave];
 #pragma unroll
   for (int j = 0; j < 6; ++j) w1[j] = rec->w1[j];
+#pragma unroll
+  for (int h = 0; h < kRegRowsPerWave; ++h) w0r[h] = rec->w0[min(max(ja + h, 0), 5)];
+  static_assert(kRegRowsPerWave == 2, "two row weights");
+  // issued with the others, not sunk into the row branches
+  asm volatile("" ::"v"(w0r[0].x), "v"(w0r[0].y), "v"(w0r[1].x), "v"(w0r[1].y));
 #pragma unroll
   for (int h = 0; h < kRegRowsPerWave; ++h) {
     const int j0 = ja + h;
     if (j0 >= 0 && j0 < J0) {
-      const float2 w0 = rec->w0[j0];
+      const float2 w0 = w0r[h];
       float2 u;
       u.x = fmaf(v.x, w0.x, -v.y * w0.y);
       u.y = fmaf(v.x, w0.y, v.y * w0.x);
