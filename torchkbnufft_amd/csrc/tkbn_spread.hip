@@ -1057,6 +1057,53 @@ __device__ __forceinline__ int64_t xcd_work_item(int64_t b) {
   return kXcdChunk * (x + 8 * (k / kXcdChunk)) + k % kXcdChunk;
 }
 
+// One entry: sum over the J0 x J1 footprint of w0[j0] * w1[j1] * region cell, for the
+// lane's coil row.  r holds the record (w0 in r[0..2], w1 in r[3..5], (l0, l1, m) in
+// r[6]).  JF > 0: J0 = J1 = JF at compile time (no per-tap branches, every LDS read
+// of the entry in flight together).
+template <int JF>
+__device__ __forceinline__ float2 gat_entry(const float4 (&r)[7], const float2* region, int row, int J0, int J1) {
+  float2 w0[6], w1[6];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    w0[2 * q] = make_float2(r[q].x, r[q].y);
+    w0[2 * q + 1] = make_float2(r[q].z, r[q].w);
+    w1[2 * q] = make_float2(r[3 + q].x, r[3 + q].y);
+    w1[2 * q + 1] = make_float2(r[3 + q].z, r[3 + q].w);
+  }
+  const int l0 = __float_as_int(r[6].x), l1 = __float_as_int(r[6].y);
+  const float2* L0 = region + (l0 * kGatR1 + l1) * kGatPitch + row;
+  float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int j0 = 0; j0 < 6; ++j0) {
+    if (JF > 0 ? j0 < JF : j0 < J0) {
+      const float2* L = L0 + j0 * (kGatR1 * kGatPitch);
+      // four independent FMA chains (real and imaginary parts of both products)
+      float2 p = make_float2(0.f, 0.f), q = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j1 = 0; j1 < 6; ++j1) {
+        if (JF > 0 ? j1 < JF : j1 < J1) {
+          const float2 w = w1[j1];
+          const float2 x = L[j1 * kGatPitch];
+          p.x = fmaf(w.x, x.x, p.x);
+          q.x = fmaf(-w.y, x.y, q.x);
+          p.y = fmaf(w.x, x.y, p.y);
+          q.y = fmaf(w.y, x.x, q.y);
+        }
+      }
+      p.x += q.x;
+      p.y += q.y;
+      const float2 w = w0[j0];
+      acc.x = fmaf(w.x, p.x, acc.x);
+      acc.x = fmaf(-w.y, p.y, acc.x);
+      acc.y = fmaf(w.x, p.y, acc.y);
+      acc.y = fmaf(w.y, p.x, acc.y);
+    }
+  }
+  return acc;
+}
+
+template <int JF>
 __global__ void __launch_bounds__(kGatThreads)
     gather_coils_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ grid,
                         const int* ws, float2* kdata, int nchunk) {  // no restrict: keeps the record loads vector loads
@@ -1111,70 +1158,33 @@ __global__ void __launch_bounds__(kGatThreads)
   __syncthreads();
 
   // Records are read with vector loads, one entry ahead of use (scalar loads would
-  // share the LDS wait counter).  Half-wave h of wave w takes entries
-  // i0 + 2w + h, stepping by 16.
+  // share the LDS wait counter), alternating between two register sets.  Half-wave h
+  // of wave w takes entries i0 + 2w + h, stepping by 16.
   const float4* recs = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) +
                        (int64_t)plan[tg.o_start + t] * 7;
   const int J0 = g.J[0], J1 = g.J[1];
   constexpr int kStep = 2 * (kGatThreads / 64);
-  float4 cur[7], nxt[7];
-  {
-    const int e = i0 + 2 * wave + half;
+  float4 ra[7], rb[7];
+  // unconditional loads (the index clamped to the last entry; i0 < i1 here), so the
+  // wait for one set does not also wait for the other set's loads
+  auto load_rec = [&](float4 (&r)[7], int e) {
+    const int ec = e < i1 ? e : i1 - 1;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) r[q] = recs[(int64_t)ec * 7 + q];
+  };
+  auto finish = [&](const float4 (&r)[7], int e) {
     if (e < i1) {
-#pragma unroll
-      for (int q = 0; q < 7; ++q) cur[q] = recs[(int64_t)e * 7 + q];
+      const float2 acc = gat_entry<JF>(r, region, row, J0, J1);
+      if (row < ncc) kdata[(rbase + row) * g.M + __float_as_int(r[6].z)] = acc;
     }
-  }
-  for (int eb = i0 + 2 * wave; eb < i1; eb += kStep) {
+  };
+  load_rec(ra, i0 + 2 * wave + half);
+  for (int eb = i0 + 2 * wave; eb < i1; eb += 2 * kStep) {
     const int e = eb + half;
-    const bool act = e < i1;
-    if (e + kStep < i1) {
-#pragma unroll
-      for (int q = 0; q < 7; ++q) nxt[q] = recs[(int64_t)(e + kStep) * 7 + q];
-    }
-    // record words: w0 in cur[0..2], w1 in cur[3..5], (l0, l1, m, pad) in cur[6]
-    float2 w0[6], w1[6];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      w0[2 * q] = make_float2(cur[q].x, cur[q].y);
-      w0[2 * q + 1] = make_float2(cur[q].z, cur[q].w);
-      w1[2 * q] = make_float2(cur[3 + q].x, cur[3 + q].y);
-      w1[2 * q + 1] = make_float2(cur[3 + q].z, cur[3 + q].w);
-    }
-    const int l0 = __float_as_int(cur[6].x), l1 = __float_as_int(cur[6].y), m = __float_as_int(cur[6].z);
-    float2 acc = make_float2(0.f, 0.f);
-    if (act) {
-      const float2* L0 = region + (l0 * kGatR1 + l1) * kGatPitch + row;
-#pragma unroll
-      for (int j0 = 0; j0 < 6; ++j0) {
-        if (j0 < J0) {
-          const float2* L = L0 + j0 * (kGatR1 * kGatPitch);
-          // four independent FMA chains (real and imaginary parts of both products)
-          float2 p = make_float2(0.f, 0.f), q = make_float2(0.f, 0.f);
-#pragma unroll
-          for (int j1 = 0; j1 < 6; ++j1) {
-            if (j1 < J1) {
-              const float2 w = w1[j1];
-              const float2 x = L[j1 * kGatPitch];
-              p.x = fmaf(w.x, x.x, p.x);
-              q.x = fmaf(-w.y, x.y, q.x);
-              p.y = fmaf(w.x, x.y, p.y);
-              q.y = fmaf(w.y, x.x, q.y);
-            }
-          }
-          p.x += q.x;
-          p.y += q.y;
-          const float2 w = w0[j0];
-          acc.x = fmaf(w.x, p.x, acc.x);
-          acc.x = fmaf(-w.y, p.y, acc.x);
-          acc.y = fmaf(w.x, p.y, acc.y);
-          acc.y = fmaf(w.y, p.x, acc.y);
-        }
-      }
-      if (row < ncc) kdata[(rbase + row) * g.M + m] = acc;
-    }
-#pragma unroll
-    for (int q = 0; q < 7; ++q) cur[q] = nxt[q];
+    load_rec(rb, e + kStep);
+    finish(ra, e);
+    load_rec(ra, e + 2 * kStep);
+    finish(rb, e + kStep);
   }
 }
 
@@ -1754,8 +1764,12 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
   const int64_t groups = ceil_div(tg.maxwork * nchunk, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
   TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_UNSUPPORTED, "too many coil rows x work items for one launch");
   profile_before(s);
-  gather_coils_kernel<<<dim3((unsigned)groups), dim3(kGatThreads), 0, s>>>(
-      g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata), nchunk);
+  if (g.J[0] == 6 && g.J[1] == 6)
+    gather_coils_kernel<6><<<dim3((unsigned)groups), dim3(kGatThreads), 0, s>>>(
+        g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata), nchunk);
+  else
+    gather_coils_kernel<0><<<dim3((unsigned)groups), dim3(kGatThreads), 0, s>>>(
+        g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata), nchunk);
   profile_after(s);
   TKBN_LAUNCH_CHECK("gather_coils_kernel");
   return TKBN_OK;
