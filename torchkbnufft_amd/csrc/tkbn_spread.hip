@@ -121,6 +121,11 @@ __device__ __forceinline__ void wave_group_by_key(int key, bool valid, int& lead
 // the wave, all issued by one instruction.  Returns the value this lane's increment
 // would have seen (lanes with equal keys are ranked in lane order), -1 if not valid.
 // Every active lane of the wave must call it.
+// Work-group barrier that orders LDS accesses only.  __syncthreads() also waits for
+// the group's outstanding global loads and stores (its workgroup-scope fence), which
+// would serialise a loop of store rounds on their completion.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 __device__ __forceinline__ int wave_aggregated_inc(int* ptr, int key, bool valid) {
   const int lane = threadIdx.x & 63;
   int leader_of, rank, cnt;
@@ -897,7 +902,7 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
 #pragma unroll
         for (int c = 0; c < kRegT1; ++c) vals[((slot0 + h) * kRegT1 + c) * kRegPitch + lane] = acc[h][c];
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < (4 * kRegT1 * 64) / kRegThreads; ++q) {
       const int e = threadIdx.x + q * kRegThreads;
@@ -914,7 +919,7 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
         }
       }
     }
-    __syncthreads();
+    lds_barrier();  // the stores stay in flight through the next rounds
   }
 }
 
