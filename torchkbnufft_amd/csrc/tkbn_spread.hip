@@ -928,18 +928,20 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
 //   The plan bins every sample once, into the tile of its base cell (the
 //   footprint's first cell), keeping trajectory order inside a tile, and stores a
 //   per-entry record: phase * w0[j], w1[j] and the base cell relative to the tile.
-//   One group (8 waves) per (8 x 16 tile, split, chunk of 32 coil rows) stages the
-//   tile and its J-1 halo (13 x 21 cells, wrapped) in LDS with the coil row fastest.
-//   Each half-wave takes one entry (lanes = the chunk's 32 rows), so a tap read by a
-//   wave is two runs of 32 consecutive LDS words and conflict-free.  Larger tiles
-//   than 64-row chunks allow stage 2.1x the tile's cells instead of 3.7x and give
-//   each staging round 4x the samples to hide its latency behind.
+//   One group (8 waves) per (16 x 16 tile, split, chunk of 16 coil rows) stages the
+//   tile and its J-1 halo (21 x 21 cells, wrapped) in LDS with the coil row fastest.
+//   Each quarter-wave takes one entry (lanes = the chunk's 16 rows), so a tap read by
+//   a wave is four runs of 16 consecutive LDS words and conflict-free.  Short row
+//   chunks allow large tiles: the region is 1.7x the tile's cells (3.7x for 4 x 8
+//   tiles of 64 rows), and each staging round covers 8x the samples, hiding its
+//   latency; the price is one record read per chunk.
 // ---------------------------------------------------------------------------
-constexpr int kGatT0 = 8, kGatT1 = 16;      // tile edges
+constexpr int kGatT0 = 16, kGatT1 = 16;     // tile edges
 constexpr int kGatR0 = kGatT0 + 5;          // staged region edges (J <= 6)
 constexpr int kGatR1 = kGatT1 + 5;
 constexpr int kGatCells = kGatR0 * kGatR1;  // 273
-constexpr int kGatRows = 32;                // coil rows per work item
+constexpr int kGatRows = 16;                // coil rows per work item
+constexpr int kGatPerWave = 64 / kGatRows;  // entries a wave takes at once
 constexpr int kGatPitch = kGatRows + 1;     // float2 per staged cell (odd: staging stores spread over banks)
 constexpr int kGatThreads = 512;
 constexpr int kGatTarget = 256;             // entries per work item
@@ -1115,7 +1117,7 @@ __global__ void __launch_bounds__(kGatThreads)
   __shared__ __attribute__((aligned(16))) float2 region[kGatCells * kGatPitch];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int half = lane >> 5, row = lane & 31;
+  const int half = lane / kGatRows, row = lane % kGatRows;  // entry slot, coil row
   const int* plan = ws;
   // the chunks of one work item are consecutive, so they meet in one XCD's L2
   const int64_t item = xcd_work_item(blockIdx.x);
@@ -1168,7 +1170,7 @@ __global__ void __launch_bounds__(kGatThreads)
   const float4* recs = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) +
                        (int64_t)plan[tg.o_start + t] * 7;
   const int J0 = g.J[0], J1 = g.J[1];
-  constexpr int kStep = 2 * (kGatThreads / 64);
+  constexpr int kStep = kGatPerWave * (kGatThreads / 64);
   float4 ra[7], rb[7];
   // unconditional loads (the index clamped to the last entry; i0 < i1 here), so the
   // wait for one set does not also wait for the other set's loads
@@ -1183,8 +1185,8 @@ __global__ void __launch_bounds__(kGatThreads)
       if (row < ncc) kdata[(rbase + row) * g.M + __float_as_int(r[6].z)] = acc;
     }
   };
-  load_rec(ra, i0 + 2 * wave + half);
-  for (int eb = i0 + 2 * wave; eb < i1; eb += 2 * kStep) {
+  load_rec(ra, i0 + kGatPerWave * wave + half);
+  for (int eb = i0 + kGatPerWave * wave; eb < i1; eb += 2 * kStep) {
     const int e = eb + half;
     load_rec(rb, e + kStep);
     finish(ra, e);
