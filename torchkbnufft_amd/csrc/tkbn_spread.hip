@@ -536,13 +536,14 @@ __global__ void __launch_bounds__(256)
 //   tile + J - 1 cells (a wrapped footprint then never meets one tile twice);
 //   smaller grids use the sample-lane kernel.
 // ---------------------------------------------------------------------------
-constexpr int kRegRowsPerWave = 2;
+constexpr int kRegRowsPerWave = 2;  // tile rows of a wave: one per half-wave
 constexpr int kRegWaves = 8;
 constexpr int kRegThreads = 64 * kRegWaves;
 constexpr int kRegT0 = kRegRowsPerWave * kRegWaves;  // 16
 constexpr int kRegT1 = 16;
+constexpr int kRegCoils = 32;  // coil rows per work item (lanes of a half-wave)
 constexpr int kRegBatch = 64;  // staged entries
-constexpr int kRegPitch = 65;  // float2 per staged entry: 64 rows + 1 (bank spread)
+constexpr int kRegPitch = kRegCoils + 1;  // float2 per staged entry (+1: bank spread)
 constexpr int kRegMinC0 = -5;  // first footprint column relative to the tile, J <= 6
 constexpr int kRegKeys = kRegT1 - kRegMinC0;  // c0 values per tile: -5 .. 15
 // A tile with more entries than kRegMaxUnsplit is split into pieces of kRegHotTarget
@@ -766,52 +767,45 @@ __device__ __forceinline__ void reg_add_row(float2 (&acc)[kRegT1], const float2 
   }
 }
 
-// one entry with footprint column C0 whose rows meet the wave's rows; the weights are
-// read from the staged record in LDS at a wave-uniform address (broadcast reads)
+// one entry with footprint column C0 for the lane's tile row (footprint row j0 of the
+// entry; outside 0 .. J0-1 the lane adds nothing).  The weights come from the staged
+// record in LDS: w1 at a wave-uniform address, w0 per half-wave; every operand is
+// read before the first use (one LDS wait per entry).
 template <int C0>
-__device__ __forceinline__ void reg_entry(float2 (&acc)[kRegRowsPerWave][kRegT1], const CoilRec* rec,
-                                          const float2 v, int ja, int J0) {
-  // every operand is read before the first use (one LDS wait per entry): the row
-  // weights at clamped indices, used only for the rows inside the footprint
-  float2 w1[6], w0r[kRegRowsPerWave];
+__device__ __forceinline__ void reg_entry(float2 (&acc)[kRegT1], const CoilRec* rec, const float2 v, int j0, int J0) {
+  float2 w1[6];
 #pragma unroll
   for (int j = 0; j < 6; ++j) w1[j] = rec->w1[j];
-#pragma unroll
-  for (int h = 0; h < kRegRowsPerWave; ++h) w0r[h] = rec->w0[min(max(ja + h, 0), 5)];
-  static_assert(kRegRowsPerWave == 2, "two row weights");
-  // issued with the others, not sunk into the row branches
-  asm volatile("" ::"v"(w0r[0].x), "v"(w0r[0].y), "v"(w0r[1].x), "v"(w0r[1].y));
-#pragma unroll
-  for (int h = 0; h < kRegRowsPerWave; ++h) {
-    const int j0 = ja + h;
-    if (j0 >= 0 && j0 < J0) {
-      const float2 w0 = w0r[h];
-      float2 u;
-      u.x = fmaf(v.x, w0.x, -v.y * w0.y);
-      u.y = fmaf(v.x, w0.y, v.y * w0.x);
-      reg_add_row<C0>(acc[h], u, w1);
-    }
+  const float2 w0 = rec->w0[min(max(j0, 0), 5)];
+  if (j0 >= 0 && j0 < J0) {
+    float2 u;
+    u.x = fmaf(v.x, w0.x, -v.y * w0.y);
+    u.y = fmaf(v.x, w0.y, v.y * w0.x);
+    reg_add_row<C0>(acc, u, w1);
   }
 }
 
-__global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_eu(6)))
     spread_coil_regs_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ kdata,
-                            const int* __restrict__ ws, float2* __restrict__ grid) {
+                            const int* __restrict__ ws, float2* __restrict__ grid, int nchunk) {
   __shared__ __attribute__((aligned(16))) float2 vals[kRegBatch * kRegPitch];
   __shared__ __attribute__((aligned(16))) CoilRec brec[kRegBatch];  // the batch's records
-  // wave-uniform in a scalar register, so the row tests below are scalar branches
+  // wave-uniform in a scalar register, so the hit tests below are scalar branches
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  const int coil = lane & (kRegCoils - 1), hh = lane / kRegCoils;  // lane = (tile row of the wave, coil)
 
   const int* plan = ws;
-  const int64_t wi = blockIdx.x;
+  // the chunks of one work item are consecutive, so hot pieces of every chunk start first
+  const int64_t wi = blockIdx.x / nchunk;
+  const int chunk = (int)(blockIdx.x - wi * nchunk);
   if (wi >= plan[tg.o_nwork]) return;
   const int* wstart = plan + tg.o_wstart;
   const int t = plan[tg.o_work + wi];
   const int split = (int)(wi - wstart[t]);
   const int64_t nrows = g.B * g.C;
-  const int64_t rbase = (int64_t)blockIdx.y * 64;
-  const int ncc = (int)((rbase + 64 < nrows ? rbase + 64 : nrows) - rbase);
+  const int64_t rbase = (int64_t)chunk * kRegCoils;
+  const int ncc = (int)((rbase + kRegCoils < nrows ? rbase + kRegCoils : nrows) - rbase);
   const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kRegT0;
   const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kRegT1;
   const int cstart = plan[tg.o_start + t * kRegKeys];
@@ -822,27 +816,26 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
   const int* perm = plan + tg.o_perm + cstart;
   const CoilRec* recs = reinterpret_cast<const CoilRec*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) + cstart;
   const int J0 = g.J[0];
-  const int myrow = wave * kRegRowsPerWave;  // first owned row, relative to the tile
+  const int wrow = wave * kRegRowsPerWave;  // the wave's first tile row
 
-  float2 acc[kRegRowsPerWave][kRegT1];
+  float2 acc[kRegT1];
 #pragma unroll
-  for (int h = 0; h < kRegRowsPerWave; ++h)
-#pragma unroll
-    for (int c = 0; c < kRegT1; ++c) acc[h][c] = make_float2(0.f, 0.f);
+  for (int c = 0; c < kRegT1; ++c) acc[c] = make_float2(0.f, 0.f);
 
   // Software pipeline: while a batch is processed, the next batch's data and records
   // are loaded into registers (vector loads: the records' first touch of memory is
   // never a wait inside the entry loop), and the trajectory indices one batch further.
   constexpr int kRecF4 = (int)(sizeof(CoilRec) / sizeof(float4));  // 7
   static_assert(kRegBatch * kRecF4 <= kRegThreads, "one record float4 per thread");
+  constexpr int kPer = kRegCoils / kRegWaves;  // coil rows a thread stages per batch
   float4 prec = make_float4(0.f, 0.f, 0.f, 0.f);
-  float2 pv[8];
+  float2 pv[kPer];
   auto prefetch = [&](int b, int mb) {
     const int n = min(kRegBatch, i1 - b);
     if ((int)threadIdx.x < n * kRecF4) prec = reinterpret_cast<const float4*>(recs + b)[threadIdx.x];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int row = wave * 8 + q;
+    for (int q = 0; q < kPer; ++q) {
+      const int row = wave * kPer + q;
       pv[q] = make_float2(0.f, 0.f);
       if (lane < n && row < ncc) pv[q] = kdata[(rbase + row) * g.M + mb];
     }
@@ -856,7 +849,7 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
   }
   for (int base = i0; base < i1; base += kRegBatch) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) vals[lane * kRegPitch + wave * 8 + q] = pv[q];
+    for (int q = 0; q < kPer; ++q) vals[lane * kRegPitch + wave * kPer + q] = pv[q];
     const int nb = min(kRegBatch, i1 - base);
     if ((int)threadIdx.x < nb * kRecF4) reinterpret_cast<float4*>(brec)[threadIdx.x] = prec;
     __syncthreads();
@@ -869,7 +862,7 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
       mnext = (b2 < i1 && lane < min(kRegBatch, i1 - b2)) ? perm[b2 + lane] : 0;
     }
     // entries whose footprint rows meet this wave's rows
-    const int jal = myrow - r0l;
+    const int jal = wrow - r0l;
     const unsigned long long hits = __ballot(jal + kRegRowsPerWave > 0 && jal < J0);
     // walk each c0 value's hits with fixed column offsets
 #define TKBN_REG_GROUP(C0)                                                       \
@@ -878,8 +871,8 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
     while (mk) {                                                                 \
       const int c = __ffsll(mk) - 1;                                             \
       mk &= mk - 1;                                                              \
-      const int ja = myrow - __builtin_amdgcn_readlane(r0l, c);                  \
-      reg_entry<C0>(acc, brec + c, vals[c * kRegPitch + lane], ja, J0);          \
+      const int ja = wrow - __builtin_amdgcn_readlane(r0l, c);                   \
+      reg_entry<C0>(acc, brec + c, vals[c * kRegPitch + coil], ja + hh, J0);     \
     }                                                                            \
   }
     TKBN_REG_GROUP(-5) TKBN_REG_GROUP(-4) TKBN_REG_GROUP(-3) TKBN_REG_GROUP(-2) TKBN_REG_GROUP(-1)
@@ -893,24 +886,23 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
 
   // Flush: every owned cell inside the grid is written (split tiles were zeroed first
   // and add).  The rows go out through LDS in four rounds of four tile rows so that
-  // each store instruction writes 16 consecutive cells (128 B) of 4 coil rows.
+  // each store instruction writes 16 consecutive cells (128 B) of 32 coil rows.
+  static_assert(4 * kRegT1 * kRegCoils <= kRegBatch * kRegPitch, "a round fits the staging area");
   for (int round = 0; round < 4; ++round) {
     if (wave >> 1 == round) {
-      const int slot0 = (wave & 1) * kRegRowsPerWave;  // rows 4*round + slot
+      const int slot = (wave & 1) * kRegRowsPerWave + hh;  // tile row 4 * round + slot
 #pragma unroll
-      for (int h = 0; h < kRegRowsPerWave; ++h)
-#pragma unroll
-        for (int c = 0; c < kRegT1; ++c) vals[((slot0 + h) * kRegT1 + c) * kRegPitch + lane] = acc[h][c];
+      for (int c = 0; c < kRegT1; ++c) vals[(slot * kRegT1 + c) * kRegPitch + coil] = acc[c];
     }
     lds_barrier();
 #pragma unroll
-    for (int q = 0; q < (4 * kRegT1 * 64) / kRegThreads; ++q) {
+    for (int q = 0; q < (4 * kRegT1 * kRegCoils) / kRegThreads; ++q) {
       const int e = threadIdx.x + q * kRegThreads;
-      const int c = e & (kRegT1 - 1), coil = (e >> 4) & 63, slot = e >> 10;
+      const int c = e & (kRegT1 - 1), cl = (e / kRegT1) & (kRegCoils - 1), slot = e / (kRegT1 * kRegCoils);
       const int k0 = o0 + 4 * round + slot, k1 = o1 + c;
-      if (coil < ncc && k0 < g.K[0] && k1 < g.K[1]) {
-        const float2 v = vals[((slot * kRegT1) + c) * kRegPitch + coil];
-        float2* dst = grid + (rbase + coil) * g.G + (int64_t)k0 * g.stride[0] + k1;
+      if (cl < ncc && k0 < g.K[0] && k1 < g.K[1]) {
+        const float2 v = vals[(slot * kRegT1 + c) * kRegPitch + cl];
+        float2* dst = grid + (rbase + cl) * g.G + (int64_t)k0 * g.stride[0] + k1;
         if (nsplit == 1) {
           *dst = v;
         } else {
@@ -939,7 +931,7 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
 constexpr int kGatT0 = 16, kGatT1 = 16;     // tile edges
 constexpr int kGatR0 = kGatT0 + 5;          // staged region edges (J <= 6)
 constexpr int kGatR1 = kGatT1 + 5;
-constexpr int kGatCells = kGatR0 * kGatR1;  // 273
+constexpr int kGatCells = kGatR0 * kGatR1;  // 441
 constexpr int kGatRows = 16;                // coil rows per work item
 constexpr int kGatPerWave = 64 / kGatRows;  // entries a wave takes at once
 constexpr int kGatPitch = kGatRows + 1;     // float2 per staged cell (odd: staging stores spread over banks)
@@ -1601,9 +1593,11 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
   coil_zero_hot_kernel<<<dim3((unsigned)tg.NB, ychunks, 1), dim3(kRegT0 * kRegT1), 0, s>>>(
       g, tg, wsi, static_cast<float2*>(grid));
   TKBN_LAUNCH_CHECK("coil_zero_hot_kernel");
+  const int nchunk = (int)ceil_div(nrows, kRegCoils);
+  TKBN_REQUIRE(tg.maxwork * nchunk < (1LL << 31), TKBN_ERR_UNSUPPORTED, "too many coil rows x work items for one launch");
   profile_before(s);
-  spread_coil_regs_kernel<<<dim3((unsigned)tg.maxwork, ychunks, 1), dim3(kRegThreads), 0, s>>>(
-      g, tg, static_cast<const float2*>(kdata), wsi, static_cast<float2*>(grid));
+  spread_coil_regs_kernel<<<dim3((unsigned)(tg.maxwork * nchunk)), dim3(kRegThreads), 0, s>>>(
+      g, tg, static_cast<const float2*>(kdata), wsi, static_cast<float2*>(grid), nchunk);
   profile_after(s);
   TKBN_LAUNCH_CHECK("spread_coil_regs_kernel");
   return TKBN_OK;
