@@ -536,12 +536,12 @@ __global__ void __launch_bounds__(256)
 //   tile + J - 1 cells (a wrapped footprint then never meets one tile twice);
 //   smaller grids use the sample-lane kernel.
 // ---------------------------------------------------------------------------
-constexpr int kRegRowsPerWave = 2;  // tile rows of a wave: one per half-wave
+constexpr int kRegCoils = 32;  // coil rows per work item (lanes of a wave slice)
+constexpr int kRegRowsPerWave = 64 / kRegCoils;  // tile rows of a wave: one per slice
 constexpr int kRegWaves = 8;
 constexpr int kRegThreads = 64 * kRegWaves;
-constexpr int kRegT0 = kRegRowsPerWave * kRegWaves;  // 16
+constexpr int kRegT0 = kRegRowsPerWave * kRegWaves;
 constexpr int kRegT1 = 16;
-constexpr int kRegCoils = 32;  // coil rows per work item (lanes of a half-wave)
 constexpr int kRegBatch = 64;  // staged entries
 constexpr int kRegPitch = kRegCoils + 1;  // float2 per staged entry (+1: bank spread)
 constexpr int kRegMinC0 = -5;  // first footprint column relative to the tile, J <= 6
@@ -747,10 +747,11 @@ __global__ void __launch_bounds__(256)
   const int64_t r1 = r0 + 64 < nrows ? r0 + 64 : nrows;
   const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kRegT0;
   const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kRegT1;
-  const int cell = threadIdx.x;  // 256 = kRegT0 * kRegT1
-  const int k0 = o0 + (cell >> 4), k1 = o1 + (cell & 15);
-  if (k0 >= g.K[0] || k1 >= g.K[1]) return;
-  for (int64_t r = r0; r < r1; ++r) grid[r * g.G + (int64_t)k0 * g.stride[0] + k1] = make_float2(0.f, 0.f);
+  for (int cell = threadIdx.x; cell < kRegT0 * kRegT1; cell += blockDim.x) {
+    const int k0 = o0 + cell / kRegT1, k1 = o1 + cell % kRegT1;
+    if (k0 >= g.K[0] || k1 >= g.K[1]) continue;
+    for (int64_t r = r0; r < r1; ++r) grid[r * g.G + (int64_t)k0 * g.stride[0] + k1] = make_float2(0.f, 0.f);
+  }
 }
 
 // acc[C0 + j] += u * w1[j] for the footprint columns that fall inside the tile
@@ -885,21 +886,24 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
   }
 
   // Flush: every owned cell inside the grid is written (split tiles were zeroed first
-  // and add).  The rows go out through LDS in four rounds of four tile rows so that
-  // each store instruction writes 16 consecutive cells (128 B) of 32 coil rows.
-  static_assert(4 * kRegT1 * kRegCoils <= kRegBatch * kRegPitch, "a round fits the staging area");
-  for (int round = 0; round < 4; ++round) {
-    if (wave >> 1 == round) {
-      const int slot = (wave & 1) * kRegRowsPerWave + hh;  // tile row 4 * round + slot
+  // and add).  The rows go out through LDS in rounds of kRound tile rows so that each
+  // store instruction writes 16 consecutive cells (128 B) of every coil row.
+  constexpr int kRound = 4;                          // tile rows per round
+  constexpr int kRoundWaves = kRound / kRegRowsPerWave;  // waves whose rows go out in a round
+  static_assert(kRound * kRegT1 * kRegCoils <= kRegBatch * kRegPitch, "a round fits the staging area");
+  static_assert(kRound % kRegRowsPerWave == 0 && (kRound * kRegT1 * kRegCoils) % kRegThreads == 0, "round shape");
+  for (int round = 0; round < kRegT0 / kRound; ++round) {
+    if (wave / kRoundWaves == round) {
+      const int slot = (wave % kRoundWaves) * kRegRowsPerWave + hh;  // tile row kRound * round + slot
 #pragma unroll
       for (int c = 0; c < kRegT1; ++c) vals[(slot * kRegT1 + c) * kRegPitch + coil] = acc[c];
     }
     lds_barrier();
 #pragma unroll
-    for (int q = 0; q < (4 * kRegT1 * kRegCoils) / kRegThreads; ++q) {
+    for (int q = 0; q < (kRound * kRegT1 * kRegCoils) / kRegThreads; ++q) {
       const int e = threadIdx.x + q * kRegThreads;
       const int c = e & (kRegT1 - 1), cl = (e / kRegT1) & (kRegCoils - 1), slot = e / (kRegT1 * kRegCoils);
-      const int k0 = o0 + 4 * round + slot, k1 = o1 + c;
+      const int k0 = o0 + kRound * round + slot, k1 = o1 + c;
       if (cl < ncc && k0 < g.K[0] && k1 < g.K[1]) {
         const float2 v = vals[(slot * kRegT1 + c) * kRegPitch + cl];
         float2* dst = grid + (rbase + cl) * g.G + (int64_t)k0 * g.stride[0] + k1;
@@ -1590,7 +1594,7 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
   }
   const int64_t nrows = g.B * g.C;
   const unsigned ychunks = (unsigned)ceil_div(nrows, 64);
-  coil_zero_hot_kernel<<<dim3((unsigned)tg.NB, ychunks, 1), dim3(kRegT0 * kRegT1), 0, s>>>(
+  coil_zero_hot_kernel<<<dim3((unsigned)tg.NB, ychunks, 1), dim3(256), 0, s>>>(
       g, tg, wsi, static_cast<float2*>(grid));
   TKBN_LAUNCH_CHECK("coil_zero_hot_kernel");
   const int nchunk = (int)ceil_div(nrows, kRegCoils);
