@@ -305,8 +305,10 @@ def bench_dcomp(dev, ktraj, iters: int, with_cpu: bool) -> dict:
 
 def bench_toeplitz(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
     """Toeplitz normal operator on the SENSE workload: one ToepNufft call replaces the
-    forward + adjoint pair; algorithmic bytes = read image + smaps, the grid once
-    through the FFT filter (written, read, kernel read), write image."""
+    forward + adjoint pair.  Algorithmic bytes are the stage boundaries of a two-pass
+    2-D FFT filter: read image + smaps, write the row-transformed intermediate
+    (B*C x N x 2N), read and write it in the column filter pass, read it and smaps
+    again in the last pass, read the kernel once, write the image."""
     import torchkbnufft_amd as tkbn
 
     t0 = time.perf_counter()
@@ -316,7 +318,8 @@ def bench_toeplitz(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
     toep = tkbn.ToepNufft()
     ms = timed(lambda: toep(x, kernel, smaps=smaps), iters)
     G2, P = (2 * N) ** 2, N * N
-    alg = B * P * 8 + B * C * P * 8 + 2 * (2 * B * C * G2 * 8) + G2 * 8 + B * C * P * 8 + B * P * 8
+    inter = B * C * N * (2 * N) * 8
+    alg = B * P * 8 + B * C * P * 8 + 4 * inter + B * C * P * 8 + G2 * 8 + B * P * 8
     return {
         "workload": f"ToepNufft on the same batch ({B} x {C} coils, {N}x{N}), kernel {2 * N}x{2 * N} from the "
         "radial trajectory; A^H A x without interpolation",

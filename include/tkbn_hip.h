@@ -33,6 +33,8 @@
  *                           norm None; "ortho" via `scale`) + modules/kbnufft.py:182-183
  *   tkbn_fft_grid_adj    <- _nufft/fft.py:80-118 ifft_and_scale including its inverse FFT
  *                           (fft.py:17-22, norm "forward"/"ortho") + modules/kbnufft.py:404-405
+ *   tkbn_fft_filter      <- _nufft/fft.py:121-173 fft_filter + modules/kbnufft.py:441-489
+ *                           (ToepNufft's SENSE loop)
  *   tkbn_interp_domega   <- no reference counterpart: the reference's autograd Functions
  *                           return None for the trajectory (_autograd/interp.py:130,178);
  *                           this gives d/d omega for both of them.
@@ -188,6 +190,22 @@ TKBN_API int tkbn_fft_grid_adj(int32_t dtype, int32_t ndim, const int64_t* im_si
                                int64_t batch, int64_t coils, const void* grid, const void* smaps,
                                const void* scaling_coef, double scale, void* image, void* workspace,
                                int64_t workspace_bytes, void* stream);
+
+/*
+ * FFT filter on an oversampled grid (the Toeplitz-embedded normal operator), replacing
+ * _nufft/fft.py:121-173 fft_filter and the batch loop of modules/kbnufft.py:441-489
+ * (ToepNufft with smaps).  complex64, ndim >= 2, sizes as for the fused FFT, the
+ * same workspace (tkbn_fft_workspace_bytes):
+ *   out[b, 0] = sum_c conj(smaps[b, c]) * crop( IFFT( kernel[b or 0] * FFT( pad_end(
+ *               image[b, c or 0] * smaps[b, c]) * scale_fwd ) ) ) * scale_adj
+ *   (smaps == NULL: out[b, c] per coil; kernel_batched: one kernel per batch item)
+ * The zero-padded grid is never written: dimension 0 is transformed, multiplied by
+ * the kernel and transformed back in one pass.
+ */
+TKBN_API int tkbn_fft_filter(int32_t dtype, int32_t ndim, const int64_t* im_size, const int64_t* grid_size,
+                             int64_t batch, int64_t coils, int64_t image_coils, const void* image,
+                             const void* smaps, const void* kernel, int32_t kernel_batched, double scale_fwd,
+                             double scale_adj, void* out, void* workspace, int64_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -133,3 +133,32 @@ class GridAdjoint(Function):
             per_coil = fft_grid_adj(grid, None, scaling_coef, ctx.im_size, ctx.scale)
             grad_smaps = _sum_to(per_coil * grad.conj(), ctx.smaps_shape)
         return grad_grid, grad_smaps, None, None, None
+
+
+class FftFilter(Function):
+    """image [, smaps] -> crop(IFFT(kernel * FFT(pad(image * smaps)))) [combined], fused.
+
+    The operator is P^T F^H D F P (with the SENSE expansion and combination around
+    it), so its adjoint is the same filter with conj(kernel): that gives the image
+    gradient.  Gradients for smaps or the kernel go through the unfused path
+    (the caller only uses this Function when neither needs one).
+    """
+
+    @staticmethod
+    def forward(ctx, image, smaps, kernel, kernel_batched, scale_fwd, scale_adj):
+        from .._nufft.fft import fft_filter_fused
+
+        ctx.args = (kernel_batched, scale_fwd, scale_adj)
+        ctx.save_for_backward(smaps, kernel)
+        return fft_filter_fused(image, smaps, kernel, kernel_batched, scale_fwd, scale_adj)
+
+    @staticmethod
+    def backward(ctx, grad):
+        from .._nufft.fft import fft_filter_fused
+
+        smaps, kernel = ctx.saved_tensors
+        kernel_batched, scale_fwd, scale_adj = ctx.args
+        grad_image = None
+        if ctx.needs_input_grad[0]:
+            grad_image = fft_filter_fused(grad, smaps, kernel.conj(), kernel_batched, scale_adj, scale_fwd)
+        return grad_image, None, None, None, None, None

@@ -83,6 +83,11 @@ _SIGNATURES = {
         [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64,
          _c_vp, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp, _c_i64, _c_vp],
     ),
+    "tkbn_fft_filter": (
+        ctypes.c_int,
+        [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64, _c_i64,
+         _c_vp, _c_vp, _c_vp, _c_i32, _c_dbl, _c_dbl, _c_vp, _c_vp, _c_i64, _c_vp],
+    ),
     "tkbn_crop_deapod": (
         ctypes.c_int,
         [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64,

@@ -251,15 +251,52 @@ def _ones_scaling(im, dtype, device) -> Tensor:
     return t
 
 
+def fft_filter_fused(image: Tensor, smaps: Optional[Tensor], kernel: Tensor, kernel_batched: bool,
+                     scale_fwd: float, scale_adj: float) -> Tensor:
+    """One tkbn_fft_filter call: image (B, 1|C, *N) [, smaps (B|1, C, *N)], kernel
+    (*K) or (B, *K) with kernel_batched; returns (B, 1|C, *N)."""
+    require_gpu(image, smaps, kernel)
+    image = image.contiguous()
+    d = image.ndim - 2
+    im = tuple(int(v) for v in image.shape[2:])
+    gs = tuple(int(v) for v in kernel.shape[-d:])
+    B = image.shape[0]
+    C = smaps.shape[1] if smaps is not None else image.shape[1]
+    if smaps is not None:
+        smaps = smaps.to(image.dtype)
+        if smaps.shape[0] != B:
+            smaps = smaps.expand(B, *smaps.shape[1:])
+        smaps = smaps.contiguous()
+        if tuple(smaps.shape) != (B, C) + im:
+            raise ValueError(f"smaps shape {tuple(smaps.shape)} does not match image {tuple(image.shape)}")
+    kernel = kernel.to(image.dtype).contiguous()
+    if kernel.numel() != (B if kernel_batched else 1) * math.prod(gs):
+        raise ValueError(f"kernel of shape {tuple(kernel.shape)} does not match batch {B} and grid {gs}")
+    out = torch.empty((B, 1 if smaps is not None else C) + im, dtype=image.dtype, device=image.device)
+    ws, nws = _fft_workspace(image.device, d, im, gs, B, C)
+    with torch.cuda.device(image.device):
+        st = _lib.load().tkbn_fft_filter(
+            _lib.dtype_code(image), d, _lib.i64_array(im), _lib.i64_array(gs), B, C, image.shape[1],
+            image.data_ptr(), smaps.data_ptr() if smaps is not None else None, kernel.data_ptr(),
+            1 if kernel_batched else 0, float(scale_fwd), float(scale_adj), out.data_ptr(), ws.data_ptr(), nws,
+            _lib.stream_of(image),
+        )
+    _lib.check(st, "tkbn_fft_filter")
+    return out
+
+
 def toeplitz_filter(image: Tensor, smaps: Optional[Tensor], kernel: Tensor, norm: Optional[str]) -> Tensor:
     """crop(iFFT(kernel * FFT(pad_end(image [* smaps])))) [summed against conj(smaps)].
 
     The grid size is the kernel's trailing extents.  Where the fused FFT covers the
-    geometry (complex64, 2/3/5-smooth sizes) the pad + FFT (+ SENSE expansion) and
-    the iFFT + crop (+ coil combine) are the fused HIP pipelines with a unit
-    scaling, and only the kernel product runs as a separate element-wise pass.
+    geometry (complex64, 2/3/5-smooth sizes, two or more dimensions) the whole
+    operator is one tkbn_fft_filter pipeline: the padded grid is never written
+    (dimension 0 is transformed, multiplied by the kernel and transformed back in one
+    pass).  One-dimensional problems, or a kernel or smaps that needs a gradient, use
+    the fused pad + FFT and iFFT + crop stages with the product in between; other
+    geometries use rocFFT.
     """
-    from .._autograd.fft import GridAdjoint, GridForward
+    from .._autograd.fft import FftFilter, GridAdjoint, GridForward
 
     if norm is not None and norm != "ortho":
         raise ValueError("Only option for norm is 'ortho'.")
@@ -268,7 +305,18 @@ def toeplitz_filter(image: Tensor, smaps: Optional[Tensor], kernel: Tensor, norm
     im = tuple(int(v) for v in image.shape[2:])
     gs = tuple(int(v) for v in kernel.shape[-d:])
     scale = _norm_scale(norm, gs)
-    if fused_supported(image.dtype, im, gs) and all(g >= n for g, n in zip(gs, im)):
+    fused = fused_supported(image.dtype, im, gs) and all(g >= n for g, n in zip(gs, im))
+    lead = tuple(kernel.shape[:-d])
+    one_pass = (
+        fused
+        and d >= 2
+        and kernel.dtype == image.dtype
+        and (lead == () or (smaps is not None and lead == (image.shape[0], 1)))
+        and not (torch.is_grad_enabled() and (kernel.requires_grad or (smaps is not None and smaps.requires_grad)))
+    )
+    if one_pass:
+        return FftFilter.apply(image, smaps, kernel, lead != (), scale, scale)
+    if fused:
         ones = _ones_scaling(im, image.dtype, image.device)
         grid = GridForward.apply(image, smaps, ones, im, gs, scale)
         return GridAdjoint.apply(grid * kernel, smaps, ones, im, scale)

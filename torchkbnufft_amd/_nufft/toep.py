@@ -22,6 +22,8 @@ from typing import Optional, Sequence, Union
 import torch
 from torch import Tensor
 
+from . import fft as _fft
+
 
 def _circular_reverse(x: Tensor, dims) -> Tensor:
     """x[..., (-m) mod n, ...] along each of ``dims``."""
@@ -90,6 +92,14 @@ def _one_kernel(
         scale = math.sqrt(math.prod(2 * n for n in im) / math.prod(int(v) for v in kgrid))
     else:
         scale = 1.0 / math.prod(2 * n for n in im)
+    shape = tuple(2 * n for n in im)
+    if _fft.fused_supported(cdtype, shape, shape):
+        # the fused HIP FFT (no padding, unit apodisation) instead of rocFFT, which
+        # compiles kernels at run time for each new length
+        if norm == "ortho":
+            scale = scale / math.sqrt(math.prod(shape))
+        ones = _fft._ones_scaling(shape, cdtype, omega.device)
+        return _fft.fft_grid_fwd(full.reshape((1, 1) + shape), None, ones, shape, shape, scale)[0, 0]
     return torch.fft.fftn(full, dim=tuple(range(d)), norm="ortho" if norm == "ortho" else None) * scale
 
 

@@ -357,6 +357,7 @@ struct Extents {
 // ---------------------------------------------------------------------------
 // last-dimension pass (contiguous lines).  One group: LINES lines of one (b, c).
 //   MODE 0: forward first pass, input = image * smaps * scaling * scale (n < N), zero pad
+// (scaling == NULL: unit apodisation, used by the FFT filter)
 //   MODE 1: adjoint last pass, output = sum_c crop * conj(scaling) * scale * conj(smaps_c)
 //   MODE 2: adjoint last pass without smaps, output = crop * conj(scaling) * scale per coil
 // In MODE 0 lines run over (b, c, n_0..n_{d-2}) with n < N; output extents
@@ -398,7 +399,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
             const int64_t n = q * N + k;
             v = src[(b * g.image_coils + (g.image_coils == 1 ? 0 : c)) * P + n];
             if (smaps) v = cmul(v, smaps[bc * P + n]);
-            v = cmul(v, scaling[n]);
+            if (scaling) v = cmul(v, scaling[n]);
             v.x *= g.scale;
             v.y *= g.scale;
           }
@@ -438,7 +439,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
         lds_fft<true, true>(buf, tw, g.plan[D - 1], nc, K, 1, g.fK);
       if constexpr (MODE == 1) {
         if (n < N) {
-          const float2 s = scaling[q * N + n];
+          const float2 s = scaling ? scaling[q * N + n] : make_float2(1.f, 0.f);
           // smaps of up to 4 coils in flight before they are used
           constexpr int kSm = 4;
           for (int l0 = 0; l0 < nc; l0 += kSm) {
@@ -464,7 +465,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
             nc * N,
             [&](int idx) {
               const int l = (int)fdiv(idx, g.fN), nn = idx - l * N;
-              float2 v = cmul_conj(buf[l * K + nn], scaling[q * N + nn]);
+              float2 v = buf[l * K + nn];
+              if (scaling) v = cmul_conj(v, scaling[q * N + nn]);
               v.x *= g.scale;
               v.y *= g.scale;
               return v;
@@ -540,6 +542,66 @@ __global__ void __launch_bounds__(1024)
       [&](int idx, float2 v) {
         const int k = divc(idx), t = idx - k * ncol;
         if (t < nc) o[(int64_t)k * inner + t] = v;
+      });
+}
+
+// ---------------------------------------------------------------------------
+// FFT filter along dimension 0 (the Toeplitz normal operator's middle): for a block
+// of adjacent columns of one (b, c) row, zero-pad the n image rows to K, transform,
+// multiply by the kernel (K x inner, one per batch item or shared), transform back
+// and keep the first n rows, in place.  The padded grid never reaches memory.
+// ---------------------------------------------------------------------------
+template <int KC>
+__global__ void __launch_bounds__(1024)
+    fft_filter_kernel(const FftPlan1D pl, int64_t rows, int n, int64_t inner, int ncol_rt, const FastDiv fcol,
+                      const float2* __restrict__ kern, int64_t rows_per_kern, float2* data) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  constexpr int NCC = KC > 0 ? mid_cols_c(KC) : 1;
+  const int K = KC > 0 ? KC : pl.K;
+  const int ncol = KC > 0 ? NCC : ncol_rt;
+  auto divc = [&](int x) { return KC > 0 ? x / NCC : (int)fdiv(x, fcol); };
+  float2* tw = reinterpret_cast<float2*>(smem);
+  float2* buf = tw + K;
+  // XCD-contiguous items, as in fft_mid_kernel (the grid is a multiple of 8)
+  const int64_t per_xcd = gridDim.x >> 3;
+  const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  const int64_t chunks = (inner + ncol - 1) / ncol;
+  const int64_t row = item / chunks;
+  const int64_t col0 = (item - row * chunks) * ncol;
+  if (row >= rows) return;
+  build_twiddles(tw, K);
+  const int nc = (int)(col0 + ncol <= inner ? ncol : inner - col0);
+  float2* d = data + row * (int64_t)n * inner + col0;
+  const float2* kr = kern + (rows_per_kern > 0 ? row / rows_per_kern : 0) * (int64_t)K * inner + col0;
+  batched_copy(
+      K * ncol,
+      [&](int idx) {
+        const int k = divc(idx), t = idx - k * ncol;
+        return (k < n && t < nc) ? d[(int64_t)k * inner + t] : make_float2(0.f, 0.f);
+      },
+      [&](int idx, float2 v) { buf[idx] = v; });
+  __syncthreads();
+  if constexpr (KC > 0)
+    lds_fft_c<false, false, KC, NCC>(buf, tw, NCC, threadIdx.x);
+  else
+    lds_fft<false, false>(buf, tw, pl, ncol, 1, ncol, fcol);
+  batched_copy(
+      K * ncol,
+      [&](int idx) {
+        const int k = divc(idx), t = idx - k * ncol;
+        return t < nc ? cmul(buf[idx], kr[(int64_t)k * inner + t]) : make_float2(0.f, 0.f);
+      },
+      [&](int idx, float2 v) { buf[idx] = v; });
+  __syncthreads();
+  if constexpr (KC > 0)
+    lds_fft_c<true, false, KC, NCC>(buf, tw, NCC, threadIdx.x);
+  else
+    lds_fft<true, false>(buf, tw, pl, ncol, 1, ncol, fcol);
+  batched_copy(
+      n * ncol, [&](int idx) { return buf[idx]; },
+      [&](int idx, float2 v) {
+        const int k = divc(idx), t = idx - k * ncol;
+        if (t < nc) d[(int64_t)k * inner + t] = v;
       });
 }
 
@@ -686,6 +748,34 @@ static int launch_mid(bool inv, const FftPlan1D& pl, int64_t rows, int n_in, int
   else
     fft_mid_kernel<false, 0><<<grid, block, lds, s>>>(pl, rows, n_in, n_out, inner, ncol, fcol, src, dst);
   TKBN_LAUNCH_CHECK("fft_mid_kernel");
+  return TKBN_OK;
+}
+
+static int launch_filter(const FftPlan1D& pl, int64_t rows, int n, int64_t inner, const float2* kern,
+                         int64_t rows_per_kern, float2* data, hipStream_t s) {
+  const int ncol = mid_cols(pl);
+  const int64_t chunks = ceil_div(inner, ncol);
+  const int64_t groups = ceil_div(rows * chunks, 8) * 8;  // a multiple of 8 (XCD mapping)
+  TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_INVALID, "too many FFT columns");
+  const size_t lds = (size_t)(pl.K + (int64_t)pl.K * ncol) * sizeof(float2);
+  const FastDiv fcol = make_fastdiv((uint32_t)ncol);
+  const dim3 grid((unsigned)groups), block(kFftThreads);
+  switch (pl.K) {
+#define TKBN_FILT(KV)                                                                                     \
+  case KV:                                                                                                \
+    if (ncol == mid_cols_c(KV)) {                                                                         \
+      fft_filter_kernel<KV><<<grid, block, lds, s>>>(pl, rows, n, inner, ncol, fcol, kern, rows_per_kern, data); \
+      TKBN_LAUNCH_CHECK("fft_filter_kernel");                                                             \
+      return TKBN_OK;                                                                                     \
+    }                                                                                                     \
+    break;
+    TKBN_FFT_COMPILED_SIZES(TKBN_FILT)
+#undef TKBN_FILT
+    default:
+      break;
+  }
+  fft_filter_kernel<0><<<grid, block, lds, s>>>(pl, rows, n, inner, ncol, fcol, kern, rows_per_kern, data);
+  TKBN_LAUNCH_CHECK("fft_filter_kernel");
   return TKBN_OK;
 }
 
@@ -856,6 +946,103 @@ int tkbn_fft_grid_adj(int32_t dtype, int32_t ndim, const int64_t* im_size, const
       const int64_t groups = g1.B * Q;
       launch_last<2>(K, (unsigned)groups, lds, s, g1, cur, nullptr, static_cast<const float2*>(scaling_coef),
                      static_cast<float2*>(image), 1);
+    }
+    TKBN_LAUNCH_CHECK("fft_last_kernel<adj>");
+  }
+  return TKBN_OK;
+}
+
+int tkbn_fft_filter(int32_t dtype, int32_t ndim, const int64_t* im_size, const int64_t* grid_size, int64_t batch,
+                    int64_t coils, int64_t image_coils, const void* image, const void* smaps, const void* kernel,
+                    int32_t kernel_batched, double scale_fwd, double scale_adj, void* out, void* workspace,
+                    int64_t workspace_bytes, void* stream) {
+  FftGeom g;
+  TKBN_REQUIRE(dtype == TKBN_F32, TKBN_ERR_UNSUPPORTED, "fused FFT is complex64 only");
+  int st = check_fft(ndim, im_size, grid_size, &g);
+  if (st != TKBN_OK) return st;
+  TKBN_REQUIRE(ndim >= 2, TKBN_ERR_UNSUPPORTED, "the fused FFT filter needs at least two dimensions");
+  TKBN_REQUIRE(batch >= 0 && coils >= 0, TKBN_ERR_INVALID, "negative batch/coils");
+  TKBN_REQUIRE(image_coils == 1 || image_coils == coils, TKBN_ERR_INVALID, "image_coils must be 1 or coils");
+  TKBN_REQUIRE(smaps != nullptr || image_coils == coils, TKBN_ERR_INVALID, "without smaps image_coils must equal coils");
+  if (batch == 0 || coils == 0) return TKBN_OK;
+  TKBN_REQUIRE(image && kernel && out, TKBN_ERR_INVALID, "NULL data pointer");
+  g.B = batch;
+  g.C = coils;
+  g.image_coils = (int)image_coils;
+  g.out_coils = smaps ? 1 : (int)coils;
+  st = set_divs(&g);
+  if (st != TKBN_OK) return st;
+  int64_t wa, wb;
+  ws_sizes(g, &wa, &wb);
+  TKBN_REQUIRE(workspace_bytes >= (wa + wb) * 8 && (wa + wb == 0 || workspace), TKBN_ERR_WORKSPACE,
+               "workspace of %lld bytes needed", (long long)((wa + wb) * 8));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int D = ndim;
+  float2* bufa = static_cast<float2*>(workspace);
+  float2* bufb = bufa + wa;
+  const int K = g.K[D - 1];
+  int64_t Q = 1;
+  for (int d = 0; d < D - 1; ++d) Q *= g.N[d];
+  // forward: last dimension (SENSE expansion, zero pad, scale_fwd), then dimensions
+  // D-2 .. 1 zero-padded; dimension 0 is transformed, filtered and transformed back
+  // in place; then dimensions 1 .. D-2 and the last one inverse with crop and coil
+  // combine (scale_adj)
+  {
+    int lpg = (int)(kFftLdsBudget / (K * 8)) - 1;
+    lpg = lpg > max_transforms(g.plan[D - 1]) ? max_transforms(g.plan[D - 1]) : lpg;
+    lpg = lpg < 1 ? 1 : (lpg > 8 ? 8 : lpg);
+    g.scale = (float)scale_fwd;
+    const int64_t groups = ceil_div(g.B * g.C * Q, lpg);
+    const size_t lds = (size_t)(K + (int64_t)lpg * K) * sizeof(float2);
+    launch_last<0>(K, (unsigned)groups, lds, s, g, static_cast<const float2*>(image),
+                   static_cast<const float2*>(smaps), nullptr, bufa, lpg);
+    TKBN_LAUNCH_CHECK("fft_last_kernel<fwd>");
+  }
+  int e[3];
+  for (int d = 0; d < D; ++d) e[d] = g.N[d];
+  e[D - 1] = g.K[D - 1];
+  float2* cur = bufa;
+  for (int d = D - 2; d >= 1; --d) {
+    int64_t outer = g.B * g.C, inner = 1;
+    for (int k = 0; k < d; ++k) outer *= e[k];
+    for (int k = d + 1; k < D; ++k) inner *= e[k];
+    float2* nxt = cur == bufa ? bufb : bufa;
+    st = launch_mid(false, g.plan[d], outer, g.N[d], g.K[d], inner, cur, nxt, s);
+    if (st != TKBN_OK) return st;
+    e[d] = g.K[d];
+    cur = nxt;
+  }
+  {
+    int64_t inner = 1;
+    for (int k = 1; k < D; ++k) inner *= e[k];
+    st = launch_filter(g.plan[0], g.B * g.C, g.N[0], inner, static_cast<const float2*>(kernel),
+                       kernel_batched ? g.C : 0, cur, s);
+    if (st != TKBN_OK) return st;
+  }
+  for (int d = 1; d <= D - 2; ++d) {
+    int64_t outer = g.B * g.C, inner = 1;
+    for (int k = 0; k < d; ++k) outer *= e[k];
+    for (int k = d + 1; k < D; ++k) inner *= e[k];
+    float2* nxt = cur == bufa ? bufb : bufa;
+    st = launch_mid(true, g.plan[d], outer, g.K[d], g.N[d], inner, cur, nxt, s);
+    if (st != TKBN_OK) return st;
+    e[d] = g.N[d];
+    cur = nxt;
+  }
+  {
+    int cc = (int)(kFftLdsBudget / (K * 8)) - 1;
+    cc = cc > max_transforms(g.plan[D - 1]) ? max_transforms(g.plan[D - 1]) : cc;
+    cc = cc < 1 ? 1 : (cc > kCopyU ? kCopyU : cc);
+    g.scale = (float)scale_adj;
+    const size_t lds = (size_t)(K + (int64_t)cc * K) * sizeof(float2);
+    if (smaps) {
+      launch_last<1>(K, (unsigned)(g.B * Q), lds, s, g, cur, static_cast<const float2*>(smaps), nullptr,
+                     static_cast<float2*>(out), cc);
+    } else {
+      FftGeom g1 = g;
+      g1.B = g.B * g.C;
+      g1.C = 1;
+      launch_last<2>(K, (unsigned)(g1.B * Q), lds, s, g1, cur, nullptr, nullptr, static_cast<float2*>(out), 1);
     }
     TKBN_LAUNCH_CHECK("fft_last_kernel<adj>");
   }
