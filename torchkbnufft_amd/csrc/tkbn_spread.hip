@@ -1313,9 +1313,9 @@ __global__ void __launch_bounds__(256)
 // busy per group; the row floor keeps 3-D tiles from holding a single row.
 static const int* tile_shape(int D, bool fwd, int64_t nrows, int64_t numpoints_max, bool f32) {
   static const int spread2[2][3] = {{32, 32, 1}, {16, 16, 1}};
-  static const int spread3[3][3] = {{16, 16, 8}, {8, 8, 8}, {8, 8, 4}};
+  static const int spread3[3][3] = {{16, 16, 12}, {8, 8, 8}, {8, 8, 4}};
   static const int fwd2[2][3] = {{32, 32, 1}, {16, 16, 1}};
-  static const int fwd3[1][3] = {{8, 8, 8}};
+  static const int fwd3[2][3] = {{16, 16, 12}, {8, 8, 8}};
   static const int one[1][3] = {{1024, 1, 1}};
   const int(*cand)[3] = one;
   int n = 1;
@@ -1324,9 +1324,12 @@ static const int* tile_shape(int D, bool fwd, int64_t nrows, int64_t numpoints_m
     n = 2;
   } else if (D == 3) {
     cand = fwd ? fwd3 : spread3;
-    n = fwd ? 1 : 3;
+    n = fwd ? 2 : 3;
   }
-  const int64_t want = nrows < 3 ? (nrows > 0 ? nrows : 1) : 3;
+  // 2-D: a tile that holds min(rows, 3) rows per group; 3-D: the largest tile that
+  // holds one row (fewer replicated entries: (16+5)^2 (12+5) / (16^2 12) = 2.4 against
+  // 4.3 for 8^3, which more than pays for the entries' weights being computed per row)
+  const int64_t want = D == 3 ? 1 : (nrows < 3 ? (nrows > 0 ? nrows : 1) : 3);
   for (int c = 0; c < n; ++c) {
     int64_t cells = 1;
     for (int i = 0; i < D; ++i) cells *= fwd ? cand[c][i] + numpoints_max - 1 : cand[c][i];
