@@ -332,6 +332,39 @@ def bench_toeplitz(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
     }
 
 
+def bench_small(dev, iters: int, with_cpu: bool) -> dict:
+    """The reference's own CPU-scale case: one coil, 256^2 -> 512^2, 402 spokes x 512
+    readout points (205 824 samples, angles s*pi/402), forward + adjoint; the CPU
+    oracle's time for the same pair when the baseline is on."""
+    import torchkbnufft_amd as tkbn
+
+    gen = torch.Generator().manual_seed(1)
+    r = -np.pi + 2 * np.pi * np.arange(512) / 512
+    th = np.arange(402) * np.pi / 402
+    om = torch.tensor(np.stack([(np.sin(th)[:, None] * r).reshape(-1), (np.cos(th)[:, None] * r).reshape(-1)]),
+                      dtype=torch.float32)
+    x = torch.complex(torch.randn(1, 1, 256, 256, generator=gen), torch.randn(1, 1, 256, 256, generator=gen))
+    fw = tkbn.KbNufft(im_size=(256, 256)).to(dev)
+    ad = tkbn.KbNufftAdjoint(im_size=(256, 256)).to(dev)
+    xd, od = x.to(dev), om.to(dev)
+    ms = timed(lambda: ad(fw(xd, od), od), iters)
+    out = {
+        "workload": "1 coil, 256x256 -> 512x512, 402 x 512 radial (205824 samples), fwd + adj",
+        "gpu_ms": ms,
+        "gpu_pairs_per_s": 1e3 / ms,
+    }
+    if with_cpu:
+        from oracle import table_path as tp
+
+        plan = tp.Plan((256, 256), dtype=torch.complex64)
+        tp.kb_table_nufft_adjoint(plan, tp.kb_table_nufft(plan, x, om), om)
+        t0 = time.perf_counter()
+        tp.kb_table_nufft_adjoint(plan, tp.kb_table_nufft(plan, x, om), om)
+        out["cpu_oracle_ms"] = (time.perf_counter() - t0) * 1e3
+        out["gpu_over_cpu"] = out["cpu_oracle_ms"] / ms
+    return out
+
+
 def bench_kooshball(dev, steps: int, warmup: int) -> dict:
     """3-D radial kooshball: 160^3 -> 320^3, 8 coils with Gaussian-smoothed maps, 3200
     spokes uniform on the sphere x 512 samples (1 638 400), deterministic spread;
@@ -533,6 +566,7 @@ def main() -> int:
         out["grad"] = bench_grad(dev, args.form_iters)
         out["dcomp"] = bench_dcomp(dev, ktraj, args.form_iters, not args.no_cpu_baseline)
         out["toeplitz"] = bench_toeplitz(dev, x, ktraj, smaps, max(args.form_iters, 10), ms)
+        out["small"] = bench_small(dev, max(args.form_iters, 20), not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, ktraj, smaps, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
