@@ -10,8 +10,12 @@ problem.  Here
   built by the first call and then reused without any plan kernels (the trajectories
   are fixed at construction), so a call only runs the gather / spread kernels of each
   problem, writing into its slice of the stacked grid (no copies);
+* the problems of a size group are spread over a few side streams (fork after the
+  group's FFT, join before the next stage), so the many small interpolation launches
+  of uneven problems run concurrently instead of one after another;
 * ``capture()`` records the whole forward + adjoint pair of all problems in a HIP
-  graph; ``replay()`` reruns it with one launch from the host.
+  graph (the fork/join becomes graph dependencies); ``replay()`` reruns it with one
+  launch from the host.
 
 Each result is the same computation as the per-problem module call (same kernels,
 same plans), so it matches the reference within the same tolerance.
@@ -37,6 +41,8 @@ class BatchedKbNufft(torch.nn.Module):
         coils: coil images per problem (each problem's image is (1, coils, *im_size)).
         numpoints, table_oversamp, kbwidth, dtype: as for ``KbNufft``.
         device: device of all buffers.
+        streams: side streams the problems of a group are dealt to (1: in order on the
+            caller's stream).  A problem always uses the same stream.
 
     Buffers (write inputs into them in place; outputs are overwritten):
         ``images[p]``  : (1, coils, *im_size_p) forward input
@@ -54,6 +60,7 @@ class BatchedKbNufft(torch.nn.Module):
         kbwidth: float = 2.34,
         dtype: torch.dtype = torch.complex64,
         device: Optional[torch.device] = None,
+        streams: int = 4,
     ):
         super().__init__()
         if len(im_sizes) != len(ktrajs):
@@ -103,6 +110,37 @@ class BatchedKbNufft(torch.nn.Module):
         self._adj_plans = [_interp.PlanHolder(trusted=True) for _ in self.sizes]
         self._graph: Optional[torch.cuda.CUDAGraph] = None
         self._stream: Optional[torch.cuda.Stream] = None
+        # side streams (default priority), created on first use on this device
+        self._nside = max(1, int(streams))
+        self._side: List[torch.cuda.Stream] = []
+        self._device = device
+
+    def _streams(self) -> List[torch.cuda.Stream]:
+        if self._nside > 1 and not self._side:
+            self._side = [torch.cuda.Stream(device=self._device) for _ in range(self._nside)]
+        return self._side
+
+    def _fanout(self, members: List[int], work) -> None:
+        """work(slot, p) for every problem of a group: problem p on side stream
+        p mod S (always the same one, as its plan holders require), after the caller's
+        stream; the caller's stream then waits for all side streams."""
+        side = self._streams()
+        if not side:
+            for slot, p in enumerate(members):
+                work(slot, p)
+            return
+        main = torch.cuda.current_stream()
+        ready = torch.cuda.Event()
+        ready.record(main)
+        for st in side:
+            st.wait_event(ready)
+        for slot, p in enumerate(members):
+            with torch.cuda.stream(side[p % len(side)]):
+                work(slot, p)
+        for st in side:
+            done = torch.cuda.Event()
+            done.record(st)
+            main.wait_event(done)
 
     def _interp_args(self, g: int):
         op = self.ops[g]
@@ -115,11 +153,14 @@ class BatchedKbNufft(torch.nn.Module):
             im, gs = self._im[g], self._gs[g]
             grid = _fft.fft_grid_fwd(self._img[g], None, op.scaling_coef, im, gs, 1.0, out=self._grid[g])
             tabs, n_shift, numpoints, L, offsets = self._interp_args(g)
-            for slot, p in enumerate(members):
+
+            def fwd(slot, p, grid=grid, tabs=tabs, n_shift=n_shift, numpoints=numpoints, L=L, offsets=offsets):
                 _interp.table_interp(
                     grid[slot : slot + 1], self.ktrajs[p], tabs, n_shift, numpoints, L, offsets,
                     out=self.kdata[p], plan=self._fwd_plans[p],
                 )
+
+            self._fanout(members, fwd)
         return self.kdata
 
     def adjoint(self) -> List[Tensor]:
@@ -129,11 +170,14 @@ class BatchedKbNufft(torch.nn.Module):
             im, gs = self._im[g], self._gs[g]
             tabs, n_shift, numpoints, L, offsets = self._interp_args(g)
             grid = self._grid[g]
-            for slot, p in enumerate(members):
+
+            def adj(slot, p, grid=grid, tabs=tabs, n_shift=n_shift, numpoints=numpoints, L=L, offsets=offsets, gs=gs):
                 _interp.table_interp_adjoint(
                     self.kdata[p], self.ktrajs[p], tabs, n_shift, numpoints, L, offsets, gs,
                     out=grid[slot : slot + 1], plan=self._adj_plans[p],
                 )
+
+            self._fanout(members, adj)
             _fft.fft_grid_adj(grid, None, op.scaling_coef, im, 1.0, out=self._adj[g])
         return self.adjoint_images
 
