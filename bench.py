@@ -332,6 +332,50 @@ def bench_toeplitz(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
     }
 
 
+def bench_spmat(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
+    """The SENSE pair with sparse-matrix interpolation (calc_tensor_spmatrix matrices,
+    exact Kaiser-Bessel weights) instead of the table.  Algorithmic bytes of one
+    interpolation: the matrix once (8-byte column + 8-byte complex value per nonzero),
+    the input vectors once and the output vectors once."""
+    import torchkbnufft_amd as tkbn
+
+    od = ktraj.to(dev)
+    M = od.shape[-1]
+    t0 = time.perf_counter()
+    mats = tkbn.calc_tensor_spmatrix(od, (N, N))
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    fw = tkbn.KbNufft(im_size=(N, N)).to(dev)
+    ad = tkbn.KbNufftAdjoint(im_size=(N, N)).to(dev)
+    t0 = time.perf_counter()
+    ad(fw(x, od, mats, smaps=smaps), od, mats, smaps=smaps)  # first call: CSR forms of both directions
+    torch.cuda.synchronize()
+    first_ms = (time.perf_counter() - t0) * 1e3
+    ms = timed(lambda: ad(fw(x, od, mats, smaps=smaps), od, mats, smaps=smaps), iters)
+    gen = torch.Generator().manual_seed(9)
+    grid = torch.complex(torch.randn(B, C, K, K, generator=gen), torch.randn(B, C, K, K, generator=gen)).to(dev)
+    data = torch.complex(torch.randn(B, C, M, generator=gen), torch.randn(B, C, M, generator=gen)).to(dev)
+    ip = tkbn.KbInterp(im_size=(N, N)).to(dev)
+    ia = tkbn.KbInterpAdjoint(im_size=(N, N)).to(dev)
+    fwd_ms = timed(lambda: ip(grid, od, mats), iters)
+    adj_ms = timed(lambda: ia(data, od, mats), iters)
+    nnz = mats[0]._nnz()
+    alg = nnz * 16 + B * C * (K * K + M) * 8
+    return {
+        "workload": f"KbNufft + KbNufftAdjoint with interp_mats on the same batch ({B} x {C} coils, {N}x{N} -> "
+        f"{K}x{K}, {M} radial samples, {nnz} nonzeros)",
+        "matrix_build_ms": build_ms,
+        "first_pair_ms": first_ms,
+        "pair_ms": ms,
+        "table_pair_ms": pair_ms,
+        "interp_fwd_ms": fwd_ms,
+        "interp_adj_ms": adj_ms,
+        "interp_algorithmic_bytes": alg,
+        "interp_fwd_hbm_frac": alg / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "interp_adj_hbm_frac": alg / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+
+
 def bench_small(dev, iters: int, with_cpu: bool) -> dict:
     """The reference's own CPU-scale case: one coil, 256^2 -> 512^2, 402 spokes x 512
     readout points (205 824 samples, angles s*pi/402), forward + adjoint; the CPU
@@ -567,6 +611,7 @@ def main() -> int:
         out["dcomp"] = bench_dcomp(dev, ktraj, args.form_iters, not args.no_cpu_baseline)
         out["toeplitz"] = bench_toeplitz(dev, x, ktraj, smaps, max(args.form_iters, 10), ms)
         out["small"] = bench_small(dev, max(args.form_iters, 20), not args.no_cpu_baseline)
+        out["spmat"] = bench_spmat(dev, x, ktraj, smaps, args.form_iters, ms)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, ktraj, smaps, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -34,6 +34,8 @@
  *   tkbn_fft_grid_adj    <- _nufft/fft.py:80-118 ifft_and_scale including its inverse FFT
  *                           (fft.py:17-22, norm "forward"/"ortho") + modules/kbnufft.py:404-405
  *   tkbn_fft_filter      <- _nufft/fft.py:121-173 fft_filter + modules/kbnufft.py:441-489
+ *   tkbn_spmv            <- _nufft/interp.py:13-85 spmat_interp / spmat_interp_adjoint
+ *                           (the torch.mm products with the matrices of _nufft/spmat.py:10-105)
  *                           (ToepNufft's SENSE loop)
  *   tkbn_interp_domega   <- no reference counterpart: the reference's autograd Functions
  *                           return None for the trajectory (_autograd/interp.py:130,178);
@@ -206,6 +208,27 @@ TKBN_API int tkbn_fft_filter(int32_t dtype, int32_t ndim, const int64_t* im_size
                              int64_t batch, int64_t coils, int64_t image_coils, const void* image,
                              const void* smaps, const void* kernel, int32_t kernel_batched, double scale_fwd,
                              double scale_adj, void* out, void* workspace, int64_t workspace_bytes, void* stream);
+
+/*
+ * Sparse-matrix interpolation, replacing the torch.mm products of
+ * _nufft/interp.py:13-85 (spmat_interp, spmat_interp_adjoint).  A is an nrows x ncols
+ * complex CSR matrix (col: int64 column of each nonzero, val: complex value, rows
+ * stored in order); x is (batch, ncols) and y is (batch, nrows), both contiguous:
+ *   y[b, r] = sum over the nonzeros e of row r of val[e] * x[b, col[e]]
+ * The rows are given as segments, seg = nseg x {row, first nonzero, end nonzero, slot}
+ * (int64, row-major): every row's nonzeros are covered by its segments in order.  A
+ * row with one segment has slot -1; the segments of a row with several take distinct
+ * slots in [0, nslots), and fix = nfix x {row, first slot, slot count} lists those
+ * rows, their slots consecutive in segment order.  Sums run in a fixed order (no
+ * atomics).  Needs tkbn_spmv_workspace_bytes of workspace (0 when batch == 1 and
+ * nslots == 0).  Columns must lie in [0, ncols): they are not checked on the device.
+ */
+TKBN_API int64_t tkbn_spmv_workspace_bytes(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch,
+                                           int64_t nslots);
+TKBN_API int tkbn_spmv(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg,
+                       const int64_t* seg, int64_t nfix, const int64_t* fix, int64_t nslots, const int64_t* col,
+                       const void* val, const void* x, void* y, void* workspace, int64_t workspace_bytes,
+                       void* stream);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,15 @@ PLAN_CASES = [
     ((50, 45), (100, 90), (5, 3), 1024, 2.34),
 ]
 
+# sparse interpolation matrices: im_size, grid_size, numpoints, order, klength, omega dtype
+SPMAT_CASES = [
+    ((12, 10), None, 6, 0.0, 40, torch.float32),
+    ((9, 14), (13, 20), (4, 5), 0.0, 33, torch.float32),
+    ((6, 5, 4), None, 4, 0.0, 20, torch.float64),
+    ((16,), (24,), 6, 1.0, 25, torch.float32),
+    ((40, 36), None, 6, 0.0, 300, torch.float32),
+]
+
 
 def noise_input(shape, gen, dtype=torch.float64):
     """``arange + N(0,1)`` (the input recipe of the reference's tests/conftest.py:28-37)."""
@@ -123,3 +132,10 @@ def grad_inputs():
     smaps = cplx((1, 4) + im, gen)
     y = cplx((1, 4, ktraj.shape[1]), gen)
     return im, ktraj, x, smaps, y
+
+
+def spmat_inputs(i):
+    """omega (d, klength) of sparse-matrix case ``i``."""
+    im, grid, J, order, m, dtype = SPMAT_CASES[i]
+    gen = torch.Generator().manual_seed(7100 + i)
+    return uniform_ktraj(len(im), m, gen, dtype)

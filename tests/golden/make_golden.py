@@ -2,7 +2,7 @@
 
 Usage (from the repository root, with the reference checkout at /root/reference):
 
-    PYTHONPATH=/root/reference python tests/golden/make_golden.py [fixtures] [c64] [plans] [grad]
+    PYTHONPATH=/root/reference python tests/golden/make_golden.py [fixtures] [c64] [plans] [grad] [spmat]
 
 Why the reference is run here
 -----------------------------
@@ -106,10 +106,31 @@ def grad_cases():
     )
 
 
+def spmat_cases():
+    out = {}
+    for i, (im, grid, J, order, m, dtype) in enumerate(gi.SPMAT_CASES):
+        omega = gi.spmat_inputs(i)
+        re, imag = tkbn.calc_tensor_spmatrix(omega, im, grid_size=grid, numpoints=J, order=order)
+        # coalesced (row-major sorted, duplicates summed): the stored form of the matrix
+        mat = torch.sparse_coo_tensor(re._indices(), torch.complex(re._values(), imag._values()), re.shape)
+        mat = mat.coalesce()
+        out[f"s{i}_indices"] = mat.indices().numpy()
+        out[f"s{i}_values"] = mat.values().numpy()
+        out[f"s{i}_shape"] = np.array(re.shape)
+        out[f"s{i}_nnz_raw"] = np.array(re._nnz())
+    np.savez_compressed(os.path.join(OUT, "spmat.npz"), **out)
+
+
 if __name__ == "__main__":
-    # optional arguments select parts: fixtures c64 plans grad (default: all)
-    parts = sys.argv[1:] or ["fixtures", "c64", "plans", "grad"]
-    for name, fn in (("fixtures", fixture_cases), ("c64", c64_cases), ("plans", plan_cases), ("grad", grad_cases)):
+    # optional arguments select parts: fixtures c64 plans grad spmat (default: all)
+    parts = sys.argv[1:] or ["fixtures", "c64", "plans", "grad", "spmat"]
+    for name, fn in (
+        ("fixtures", fixture_cases),
+        ("c64", c64_cases),
+        ("plans", plan_cases),
+        ("grad", grad_cases),
+        ("spmat", spmat_cases),
+    ):
         if name in parts:
             fn()
     for f in sorted(os.listdir(OUT)):

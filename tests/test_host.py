@@ -135,8 +135,10 @@ def test_python_validation_matches_reference_errors():
     nu = tkbn.KbNufft(im_size=(8, 8))
     with pytest.raises(TypeError, match="smaps"):
         nu(image, torch.zeros(2, 5), smaps=torch.randn(1, 2, 8, 8, dtype=torch.complex128))
-    with pytest.raises(NotImplementedError):
-        nu(image, torch.zeros(2, 5), interp_mats=(torch.zeros(1), torch.zeros(1)))
+    with pytest.raises(TypeError, match="2-tuple"):
+        tkbn.KbInterpAdjoint(im_size=(8, 8))(
+            torch.randn(1, 1, 5, dtype=torch.complex64), torch.zeros(2, 5), interp_mats=torch.zeros(1)
+        )
 
 
 def test_cpu_tensors_are_refused_not_emulated():

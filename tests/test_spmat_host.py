@@ -1,0 +1,116 @@
+"""Sparse interpolation matrices and their CSR segment plan (CPU).
+
+* ``calc_tensor_spmatrix`` against the reference's own output (golden vectors from
+  ``torchkbnufft/_nufft/spmat.py:10-105``, coalesced): indices and values bit-exact,
+  including the float32 grid-coordinate arithmetic of a float32 trajectory;
+* the matrix against the oracle's closed-form exact Kaiser-Bessel interpolation
+  (``oracle.nudft.exact_kb_interp``, an independent per-tap evaluation) for float64
+  trajectories;
+* the row-segment plan that ``tkbn_spmv`` consumes: segments cover every row in
+  order, and the segment / fix-up sums it prescribes reproduce the dense product
+  (emulated here in numpy, so no GPU is needed).
+"""
+import numpy as np
+import pytest
+import torch
+
+import inputs as gi
+import torchkbnufft_amd as tkbn
+from conftest import golden
+from oracle import nudft
+from torchkbnufft_amd._nufft import spmat as sp
+
+
+def _dense(mats):
+    re, im = mats
+    return (re.to_dense() + 1j * im.to_dense()).to(torch.complex128).numpy()
+
+
+@pytest.mark.parametrize("i", range(len(gi.SPMAT_CASES)))
+def test_spmatrix_matches_reference_golden(i):
+    g = golden("spmat.npz")
+    im, grid, J, order, m, dtype = gi.SPMAT_CASES[i]
+    re, imag = tkbn.calc_tensor_spmatrix(gi.spmat_inputs(i), im, grid_size=grid, numpoints=J, order=order)
+    assert re.dtype == dtype and tuple(re.shape) == tuple(g[f"s{i}_shape"])
+    assert re._nnz() == int(g[f"s{i}_nnz_raw"])
+    mat = torch.sparse_coo_tensor(re._indices(), torch.complex(re._values(), imag._values()), re.shape).coalesce()
+    np.testing.assert_array_equal(mat.indices().numpy(), g[f"s{i}_indices"])
+    np.testing.assert_array_equal(mat.values().numpy(), g[f"s{i}_values"])
+
+
+@pytest.mark.parametrize(
+    "im_size,grid_size,numpoints,n_shift",
+    [((10,), (23,), 6, None), ((12, 9), (20, 16), (4, 6), (3, 5)), ((6, 5, 7), None, 3, None), ((16, 16), None, 6, None)],
+)
+def test_spmatrix_is_exact_kb_operator(im_size, grid_size, numpoints, n_shift):
+    gen = torch.Generator().manual_seed(5)
+    d = len(im_size)
+    om = gi.uniform_ktraj(d, 57, gen)
+    mats = tkbn.calc_tensor_spmatrix(om, im_size, grid_size=grid_size, numpoints=numpoints, n_shift=n_shift)
+    gs = grid_size or tuple(2 * n for n in im_size)
+    J = (numpoints,) * d if isinstance(numpoints, int) else numpoints
+    ns = n_shift or tuple(n // 2 for n in im_size)
+    grid = np.random.default_rng(1).standard_normal((2,) + gs) + 1j * np.random.default_rng(2).standard_normal((2,) + gs)
+    got = grid.reshape(2, -1) @ _dense(mats).T
+    ref = nudft.exact_kb_interp(grid, om.numpy(), im_size, J, [2.34 * j for j in J], ns)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-13
+
+
+def test_spmatrix_errors():
+    with pytest.raises(ValueError, match="batched omega"):
+        tkbn.calc_tensor_spmatrix(torch.zeros(2, 2, 5), (8, 8))
+    ob = tkbn.KbInterp(im_size=(8, 8))
+    with pytest.raises(TypeError, match="2-tuple"):
+        ob(torch.zeros(1, 1, 16, 16, dtype=torch.complex64), torch.zeros(2, 5), interp_mats=[torch.zeros(1)] * 2)
+
+
+def _emulate(plan, x):
+    """The sums tkbn_spmv performs, from the plan's arrays (x: (batch, ncols))."""
+    seg, fix = plan.seg.numpy(), plan.fix.numpy()
+    col, val = plan.col.numpy(), plan.val.numpy()
+    yt = np.full((plan.nrows, x.shape[0]), np.nan, dtype=np.complex128)
+    part = np.full((plan.nslots, x.shape[0]), np.nan, dtype=np.complex128)
+    for row, e0, e1, slot in seg:
+        s = (val[e0:e1, None] * x[:, col[e0:e1]].T).sum(0)
+        if slot < 0:
+            yt[row] = s
+        else:
+            part[slot] = s
+    for row, first, cnt in fix:
+        yt[row] = part[first : first + cnt].sum(0)
+    return yt.T
+
+
+def test_segment_plan_covers_rows_and_reproduces_product():
+    rng = np.random.default_rng(3)
+    nrows, ncols = 40, 5000
+    counts = rng.integers(0, 12, nrows)
+    counts[[3, 17]] = [700, 300]  # rows split into segments; one empty row below
+    counts[25] = 0
+    rows = np.repeat(np.arange(nrows), counts)
+    cols = rng.integers(0, ncols, rows.size)  # duplicates included
+    vals = rng.standard_normal(rows.size) + 1j * rng.standard_normal(rows.size)
+    perm = rng.permutation(rows.size)
+    rows, cols, vals = rows[perm], cols[perm], vals[perm]
+    plan = sp.SpmvPlan(torch.tensor(rows), torch.tensor(cols), torch.tensor(vals), nrows, ncols)
+    seg = plan.seg.numpy()
+    # segments: row-ordered, contiguous per row, at most _SEG_NNZ nonzeros, covering crow
+    assert np.all(np.diff(seg[:, 0]) >= 0) and set(seg[:, 0]) == set(range(nrows))
+    assert np.all(seg[:, 2] - seg[:, 1] <= sp._SEG_NNZ) and np.all(seg[:, 2] >= seg[:, 1])
+    for r in range(nrows):
+        s = seg[seg[:, 0] == r]
+        assert np.all(s[1:, 1] == s[:-1, 2])
+        assert (s[:, 3] < 0).all() == (len(s) == 1)
+    uniq = np.array([np.unique(cols[rows == r]).size for r in range(nrows)])
+    nseg = np.maximum(1, -(-uniq // sp._SEG_NNZ))
+    assert plan.nseg == nseg.sum() and plan.nfix == (nseg > 1).sum() >= 1
+    assert plan.nslots == nseg[nseg > 1].sum()
+    dense = np.zeros((nrows, ncols), dtype=np.complex128)
+    np.add.at(dense, (rows, cols), vals)
+    x = rng.standard_normal((3, ncols)) + 1j * rng.standard_normal((3, ncols))
+    np.testing.assert_allclose(_emulate(plan, x), x @ dense.T, rtol=1e-12, atol=1e-12)
+
+
+def test_plan_index_check():
+    with pytest.raises(ValueError, match="outside"):
+        sp.SpmvPlan(torch.tensor([0, 5]), torch.tensor([1, 2]), torch.ones(2, dtype=torch.complex64), 4, 4)

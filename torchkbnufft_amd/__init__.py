@@ -13,6 +13,7 @@ from .batched import BatchedKbNufft
 from ._math import absolute, complex_mult, complex_sign, conj_complex_mult, imag_exp, inner_product
 from ._nufft import utils as nufft_utils
 from ._nufft.dcomp import calc_density_compensation_function
+from ._nufft.spmat import calc_tensor_spmatrix
 from ._nufft.toep import calc_toeplitz_kernel
 from ._nufft.interp import clear_plan_cache, deterministic_enabled, set_deterministic
 from .modules import KbInterp, KbInterpAdjoint, KbNufft, KbNufftAdjoint, ToepNufft
@@ -27,6 +28,7 @@ __all__ = [
     "ToepNufft",
     "absolute",
     "calc_density_compensation_function",
+    "calc_tensor_spmatrix",
     "calc_toeplitz_kernel",
     "complex_mult",
     "complex_sign",

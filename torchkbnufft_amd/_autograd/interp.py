@@ -11,6 +11,7 @@ import torch
 from torch.autograd import Function
 
 from .._nufft import interp as _interp
+from .._nufft import spmat as _spmat
 
 
 class KbTableInterpForward(Function):
@@ -62,3 +63,30 @@ class KbTableInterpAdjoint(Function):
                 grad.contiguous(), data, omega, ctx.tables, n_shift, numpoints, table_oversamp
             )
         return grad_data, grad_omega, None, None, None, None, None, None
+
+
+class KbSpmatInterpForward(Function):
+    """Sparse-matrix interpolation (``torchkbnufft/_autograd/interp.py:12-43``)."""
+
+    @staticmethod
+    def forward(ctx, image, interp_mats):
+        ctx.interp_mats = interp_mats
+        ctx.grid_size = tuple(image.shape[2:])
+        return _spmat.spmat_interp(image, interp_mats)
+
+    @staticmethod
+    def backward(ctx, data):
+        return _spmat.spmat_interp_adjoint(data, ctx.interp_mats, ctx.grid_size), None
+
+
+class KbSpmatInterpAdjoint(Function):
+    """Sparse-matrix interpolation adjoint (``torchkbnufft/_autograd/interp.py:46-75``)."""
+
+    @staticmethod
+    def forward(ctx, data, interp_mats, grid_size):
+        ctx.interp_mats = interp_mats
+        return _spmat.spmat_interp_adjoint(data, interp_mats, grid_size)
+
+    @staticmethod
+    def backward(ctx, image):
+        return _spmat.spmat_interp(image, ctx.interp_mats), None, None

@@ -24,7 +24,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libtkbn_hip.so")
 
-SOURCES = ["tkbn_interp.hip", "tkbn_spread.hip", "tkbn_fftscale.hip", "tkbn_fft.hip"]
+SOURCES = ["tkbn_interp.hip", "tkbn_spread.hip", "tkbn_fftscale.hip", "tkbn_fft.hip", "tkbn_spmat.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

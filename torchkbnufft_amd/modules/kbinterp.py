@@ -1,4 +1,8 @@
-"""Interpolation modules (``torchkbnufft/modules/kbinterp.py:10-294``), table mode."""
+"""Interpolation modules (``torchkbnufft/modules/kbinterp.py:10-294``).
+
+Table interpolation by default; with ``interp_mats`` (from ``calc_tensor_spmatrix``)
+the sparse-matrix path, as in the reference (``kbinterp.py:137-138, 269-272``).
+"""
 from __future__ import annotations
 
 from typing import Optional, Tuple
@@ -8,14 +12,6 @@ from torch import Tensor
 
 from .. import functional as F
 from ._kbmodule import KbModule
-
-
-def _no_spmat(interp_mats):
-    if interp_mats is not None:
-        raise NotImplementedError(
-            "sparse-matrix interpolation (interp_mats) is not part of torchkbnufft_amd; "
-            "the table path is used"
-        )
 
 
 class KbInterpModule(KbModule):
@@ -32,7 +28,8 @@ class KbInterp(KbInterpModule):
 
     def forward(self, image: Tensor, omega: Tensor, interp_mats: Optional[Tuple[Tensor, Tensor]] = None) -> Tensor:
         """``image`` (N, C, *grid) -> (N, C, klength); ``omega`` (d, klength) or (N, d, klength)."""
-        _no_spmat(interp_mats)
+        if interp_mats is not None:
+            return F.kb_spmat_interp(image=image, interp_mats=interp_mats)
         return F.kb_table_interp(
             image=image,
             omega=omega,
@@ -55,9 +52,10 @@ class KbInterpAdjoint(KbInterpModule):
         grid_size: Optional[Tensor] = None,
     ) -> Tensor:
         """``data`` (N, C, klength) -> (N, C, *grid_size)."""
-        _no_spmat(interp_mats)
         if grid_size is None:
             grid_size = self.grid_size
+        if interp_mats is not None:
+            return F.kb_spmat_interp_adjoint(data=data, interp_mats=interp_mats, grid_size=grid_size)
         return F.kb_table_interp_adjoint(
             data=data,
             omega=omega,

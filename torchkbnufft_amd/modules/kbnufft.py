@@ -2,7 +2,9 @@
 
 SENSE expansion and coil combination are fused into the HIP scale/pad and
 crop/deapodise passes instead of separate ``image * smaps`` and
-``sum(out * conj(smaps))`` ATen ops.
+``sum(out * conj(smaps))`` ATen ops.  With ``interp_mats`` (from
+``calc_tensor_spmatrix``) the interpolation is the sparse-matrix path
+(``kbnufft.py:185-197, 364-376``).
 """
 from __future__ import annotations
 
@@ -11,10 +13,9 @@ from typing import Optional, Sequence, Tuple, Union
 import torch
 from torch import Tensor
 
-from ..functional.nufft import _nufft_adjoint, _nufft_forward
+from ..functional.nufft import _nufft_adjoint, _nufft_forward, _spmat_nufft_adjoint, _spmat_nufft_forward
 from .._nufft.utils import compute_scaling_coefs
 from ._kbmodule import KbModule
-from .kbinterp import _no_spmat
 
 
 class KbNufftModule(KbModule):
@@ -80,8 +81,10 @@ class KbNufft(KbNufftModule):
         smaps: Optional[Tensor] = None,
         norm: Optional[str] = None,
     ) -> Tensor:
-        _no_spmat(interp_mats)
         image, smaps, is_complex = _complex_view(image, smaps)
+        if interp_mats is not None:
+            out = _spmat_nufft_forward(image, smaps, self.scaling_coef, self.im_size, self.grid_size, interp_mats, norm)
+            return out if is_complex else torch.view_as_real(out)
         out = _nufft_forward(
             image, smaps, self.scaling_coef, self.im_size, self.grid_size, omega, self._tables(),
             self.n_shift, self.numpoints, self.table_oversamp, self.offsets.to(torch.long), norm,
@@ -103,10 +106,12 @@ class KbNufftAdjoint(KbNufftModule):
         smaps: Optional[Tensor] = None,
         norm: Optional[str] = None,
     ) -> Tensor:
-        _no_spmat(interp_mats)
         if smaps is not None and not smaps.dtype == data.dtype:
             raise TypeError("data dtype does not match smaps dtype.")
         data, smaps, is_complex = _complex_view(data, smaps)
+        if interp_mats is not None:
+            out = _spmat_nufft_adjoint(data, smaps, self.scaling_coef, self.im_size, self.grid_size, interp_mats, norm)
+            return out if is_complex else torch.view_as_real(out)
         out = _nufft_adjoint(
             data, smaps, self.scaling_coef, self.im_size, self.grid_size, omega, self._tables(),
             self.n_shift, self.numpoints, self.table_oversamp, self.offsets.to(torch.long), norm,
