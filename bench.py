@@ -335,8 +335,9 @@ def bench_toeplitz(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
 def bench_spmat(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
     """The SENSE pair with sparse-matrix interpolation (calc_tensor_spmatrix matrices,
     exact Kaiser-Bessel weights) instead of the table.  Algorithmic bytes of one
-    interpolation: the matrix once (8-byte column + 8-byte complex value per nonzero),
-    the input vectors once and the output vectors once."""
+    interpolation: the matrix once (4-byte column + 8-byte complex value per nonzero),
+    the input vectors once and the output vectors once.  Also the interpolations alone
+    for one coil, sparse matrix and table."""
     import torchkbnufft_amd as tkbn
 
     od = ktraj.to(dev)
@@ -359,8 +360,15 @@ def bench_spmat(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
     ia = tkbn.KbInterpAdjoint(im_size=(N, N)).to(dev)
     fwd_ms = timed(lambda: ip(grid, od, mats), iters)
     adj_ms = timed(lambda: ia(data, od, mats), iters)
+    g1, d1 = grid[:1, :1].contiguous(), data[:1, :1].contiguous()
+    one = {
+        "spmat_fwd_ms": timed(lambda: ip(g1, od, mats), iters),
+        "spmat_adj_ms": timed(lambda: ia(d1, od, mats), iters),
+        "table_fwd_ms": timed(lambda: ip(g1, od), iters),
+        "table_adj_ms": timed(lambda: ia(d1, od), iters),
+    }
     nnz = mats[0]._nnz()
-    alg = nnz * 16 + B * C * (K * K + M) * 8
+    alg = nnz * 12 + B * C * (K * K + M) * 8
     return {
         "workload": f"KbNufft + KbNufftAdjoint with interp_mats on the same batch ({B} x {C} coils, {N}x{N} -> "
         f"{K}x{K}, {M} radial samples, {nnz} nonzeros)",
@@ -373,6 +381,7 @@ def bench_spmat(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
         "interp_algorithmic_bytes": alg,
         "interp_fwd_hbm_frac": alg / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "interp_adj_hbm_frac": alg / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "one_coil_interp": one,
     }
 
 
@@ -503,9 +512,10 @@ def main() -> int:
     ap.add_argument("--stage-iters", type=int, default=20)
     ap.add_argument("--form-iters", type=int, default=5, help="iterations of the batched / gradient forms")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
-    ap.add_argument("--workload", default="sense2d", choices=["sense2d", "kooshball3d", "copy512"],
+    ap.add_argument("--workload", default="sense2d", choices=["sense2d", "kooshball3d", "copy512", "spmat"],
                     help="sense2d: the headline SENSE pair (default); kooshball3d: the 3-D deterministic case; "
-                    "copy512: a plain 512 MiB read + 512 MiB write kernel (calibrates rocprofv3 FETCH/WRITE_SIZE)")
+                    "copy512: a plain 512 MiB read + 512 MiB write kernel (calibrates rocprofv3 FETCH/WRITE_SIZE); "
+                    "spmat: the SENSE batch with sparse-matrix interpolation only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -538,6 +548,9 @@ def main() -> int:
     smaps = make_smaps(gen, dev)
     fw = tkbn.KbNufft(im_size=(N, N), numpoints=J, table_oversamp=L, dtype=torch.complex64).to(dev)
     ad = tkbn.KbNufftAdjoint(im_size=(N, N), numpoints=J, table_oversamp=L, dtype=torch.complex64).to(dev)
+    if args.workload == "spmat":
+        print(json.dumps(bench_spmat(dev, x, ktraj, smaps, args.form_iters, None)), flush=True)
+        return 0
 
     def step():
         kd = fw(x, ktraj, smaps=smaps)

@@ -212,7 +212,7 @@ TKBN_API int tkbn_fft_filter(int32_t dtype, int32_t ndim, const int64_t* im_size
 /*
  * Sparse-matrix interpolation, replacing the torch.mm products of
  * _nufft/interp.py:13-85 (spmat_interp, spmat_interp_adjoint).  A is an nrows x ncols
- * complex CSR matrix (col: int64 column of each nonzero, val: complex value, rows
+ * complex CSR matrix (col: int32 column of each nonzero, val: complex value, rows
  * stored in order); x is (batch, ncols) and y is (batch, nrows), both contiguous:
  *   y[b, r] = sum over the nonzeros e of row r of val[e] * x[b, col[e]]
  * The rows are given as segments, seg = nseg x {row, first nonzero, end nonzero, slot}
@@ -221,12 +221,13 @@ TKBN_API int tkbn_fft_filter(int32_t dtype, int32_t ndim, const int64_t* im_size
  * slots in [0, nslots), and fix = nfix x {row, first slot, slot count} lists those
  * rows, their slots consecutive in segment order.  Sums run in a fixed order (no
  * atomics).  Needs tkbn_spmv_workspace_bytes of workspace (0 when batch == 1 and
- * nslots == 0).  Columns must lie in [0, ncols): they are not checked on the device.
+ * nslots == 0).  Columns must lie in [0, ncols), ncols < 2^31: they are not checked
+ * on the device.
  */
 TKBN_API int64_t tkbn_spmv_workspace_bytes(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch,
                                            int64_t nslots);
 TKBN_API int tkbn_spmv(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg,
-                       const int64_t* seg, int64_t nfix, const int64_t* fix, int64_t nslots, const int64_t* col,
+                       const int64_t* seg, int64_t nfix, const int64_t* fix, int64_t nslots, const int32_t* col,
                        const void* val, const void* x, void* y, void* workspace, int64_t workspace_bytes,
                        void* stream);
 

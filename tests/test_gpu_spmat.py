@@ -215,3 +215,23 @@ def test_spmat_radial_config_against_table_path():
     rhs = torch.vdot(grid.reshape(-1).to(torch.complex128), x_sp.reshape(-1).to(torch.complex128))
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
     assert torch.equal(ad(data, ktraj, mats), x_sp)
+
+
+@pytest.mark.parametrize("shared", [True, False])
+def test_spmv_user_matrices(shared):
+    """Hand-built (real, imaginary) pairs: duplicates, empty rows, different patterns."""
+    print(f"[case] spmv user matrices shared={shared}", flush=True)
+    rng = np.random.default_rng(8)
+    nrows, ncols = 300, 12 * 10
+    ri = np.stack([rng.integers(0, nrows - 20, 5000), rng.integers(0, ncols, 5000)])  # last rows empty
+    ii = ri.copy() if shared else np.stack([rng.integers(0, nrows, 3000), rng.integers(0, ncols, 3000)])
+    re = torch.sparse_coo_tensor(torch.tensor(ri), torch.tensor(rng.standard_normal(ri.shape[1])), (nrows, ncols))
+    im = torch.sparse_coo_tensor(torch.tensor(ii), torch.tensor(rng.standard_normal(ii.shape[1])), (nrows, ncols))
+    A = re.to_dense().numpy() + 1j * im.to_dense().numpy()
+    mats = (re.to(DEV), im.to(DEV))
+    grid = torch.complex(torch.randn(2, 3, 12, 10, dtype=torch.float64), torch.randn(2, 3, 12, 10, dtype=torch.float64))
+    data = torch.complex(torch.randn(2, 3, nrows, dtype=torch.float64), torch.randn(2, 3, nrows, dtype=torch.float64))
+    y = tkbn.functional.kb_spmat_interp(grid.to(DEV), mats)
+    x = tkbn.functional.kb_spmat_interp_adjoint(data.to(DEV), mats, torch.tensor([12, 10]))
+    assert rel_l2(y.reshape(6, -1), torch.from_numpy(grid.reshape(6, -1).numpy() @ A.T)) < 1e-13
+    assert rel_l2(x.reshape(6, -1), torch.from_numpy(data.reshape(6, -1).numpy() @ A.conj())) < 1e-13

@@ -114,3 +114,24 @@ def test_segment_plan_covers_rows_and_reproduces_product():
 def test_plan_index_check():
     with pytest.raises(ValueError, match="outside"):
         sp.SpmvPlan(torch.tensor([0, 5]), torch.tensor([1, 2]), torch.ones(2, dtype=torch.complex64), 4, 4)
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+@pytest.mark.parametrize("shared", [True, False])
+def test_plan_merges_real_and_imaginary_parts(adjoint, shared):
+    """A = A_re + i A_im (conjugate transpose for the adjoint), also when the two parts
+    have different sparsity patterns or duplicate entries."""
+    rng = np.random.default_rng(8)
+    nrows, ncols, nnz = 30, 50, 400
+    ri = np.stack([rng.integers(0, nrows, nnz), rng.integers(0, ncols, nnz)])
+    ii = ri.copy() if shared else np.stack([rng.integers(0, nrows, 300), rng.integers(0, ncols, 300)])
+    rv, iv = rng.standard_normal(ri.shape[1]), rng.standard_normal(ii.shape[1])
+    re = torch.sparse_coo_tensor(torch.tensor(ri), torch.tensor(rv), (nrows, ncols))
+    im = torch.sparse_coo_tensor(torch.tensor(ii), torch.tensor(iv), (nrows, ncols))
+    dense = re.to_dense().numpy() + 1j * im.to_dense().numpy()
+    if adjoint:
+        dense = dense.conj().T
+    plan = sp._plan((re, im), adjoint, torch.empty(1, dtype=torch.complex128))
+    assert (plan.nrows, plan.ncols) == dense.shape
+    x = rng.standard_normal((2, dense.shape[1])) + 1j * rng.standard_normal((2, dense.shape[1]))
+    np.testing.assert_allclose(_emulate(plan, x), x @ dense.T, rtol=1e-12, atol=1e-12)

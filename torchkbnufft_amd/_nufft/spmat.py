@@ -135,7 +135,9 @@ class SpmvPlan:
             summed = torch.zeros((uniq.numel(), 2), dtype=vals.real.dtype, device=dev)
             vals = torch.view_as_complex(summed.index_add_(0, inverse, torch.view_as_real(vals)))
         r = torch.div(uniq, ncols, rounding_mode="floor")
-        self.col = (uniq - r * ncols).contiguous()
+        if ncols >= 2**31:
+            raise ValueError("sparse interpolation supports at most 2^31 - 1 grid points")
+        self.col = (uniq - r * ncols).to(torch.int32).contiguous()
         self.val = vals.contiguous()
         crow = torch.zeros(self.nrows + 1, dtype=torch.long, device=dev)
         crow[1:] = torch.cumsum(torch.bincount(r, minlength=self.nrows), 0)
@@ -209,13 +211,15 @@ def _plan(interp_mats, adjoint: bool, like: Tensor) -> SpmvPlan:
     if real not in (torch.float32, torch.float64):
         raise TypeError(f"interp_mats values must be float32 or float64, got {real}")
     dev = like.device
-    rows = torch.cat([ri[0], ii[0]]).to(dev)
-    cols = torch.cat([ri[1], ii[1]]).to(dev)
-    z_r = torch.zeros_like(rv, dtype=rt, device=dev)
-    z_i = torch.zeros_like(iv, dtype=rt, device=dev)
-    vals = torch.cat(
-        [torch.complex(rv.to(dev, rt), z_r), torch.complex(z_i, iv.to(dev, rt))]
-    )  # A = A_re + i A_im
+    ri, ii = ri.to(dev), ii.to(dev)
+    if ri.shape == ii.shape and torch.equal(ri, ii):  # calc_tensor_spmatrix's pairs share indices
+        rows, cols = ri[0], ri[1]
+        vals = torch.complex(rv.to(dev, rt), iv.to(dev, rt))
+    else:  # A = A_re + i A_im, entries summed when the CSR form is built
+        rows, cols = torch.cat([ri[0], ii[0]]), torch.cat([ri[1], ii[1]])
+        z_r = torch.zeros_like(rv, dtype=rt, device=dev)
+        z_i = torch.zeros_like(iv, dtype=rt, device=dev)
+        vals = torch.cat([torch.complex(rv.to(dev, rt), z_r), torch.complex(z_i, iv.to(dev, rt))])
     nrows, ncols = int(rshape[0]), int(rshape[1])
     if adjoint:  # conjugate transpose
         rows, cols, vals, nrows, ncols = cols, rows, vals.conj(), ncols, nrows

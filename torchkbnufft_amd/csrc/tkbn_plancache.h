@@ -6,12 +6,23 @@
 // bitwise the ones the plan was built from, and the plan kernels then exit at once.
 // Everything stays on the stream: no host synchronisation, no allocation.
 //
+// One check kernel per call, tagged with a call number unique in the process: block 0
+// records whether the stored key matches (and stores the new key), every block
+// compares its share of the words with the copy (storing the new words over it) and
+// writes the call number into `stale` on a difference.  The plan kernels that follow
+// on the stream reuse the plan when the key matched and `stale` does not hold this
+// call's number (cache_hit).  A call whose plan kernels cannot all be enqueued marks
+// the header stale (plan_cache_invalidate).  Replaying a captured non-reusing call
+// keeps its call number: after one replay that found a difference, later replays
+// rebuild (never reuse a stale plan).
+//
 // Workspace layout: [header, 256 B][copy of omega and tables][plan ...].
 // The caller zeroes the header before the workspace's first use (a zero header never
 // matches); afterwards the workspace is passed unchanged.
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 
 #include "tkbn_common.h"
 
@@ -20,10 +31,15 @@ namespace tkbn {
 constexpr int64_t kCacheHdrBytes = 256;
 
 struct CacheHdr {
-  int valid;               // the plan and the copy below belong to `key`
-  int ok;                  // this call reuses the plan (set by begin / compare)
-  unsigned long long key;  // geometry fingerprint
+  int valid;                 // the copy below and the plan belong to `key`
+  int key_ok;                // this call's key matched a valid stored key (set by block 0)
+  unsigned long long key;    // geometry fingerprint
+  unsigned long long call;   // number of the current call
+  unsigned long long stale;  // number of the last call that found a word changed
 };
+
+// the current call reuses the plan
+__device__ __forceinline__ bool cache_hit(const CacheHdr* h) { return h->key_ok && h->stale != h->call; }
 
 // up to four word ranges compared against / copied into the cache (omega, tables)
 struct CacheSrc {
@@ -31,10 +47,6 @@ struct CacheSrc {
   int64_t n[4];  // 32-bit words
   int64_t total;
 };
-
-__global__ void cache_begin_kernel(CacheHdr* h, unsigned long long key) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) h->ok = (h->valid == 1 && h->key == key) ? 1 : 0;
-}
 
 __device__ __forceinline__ unsigned int cache_src_word(const CacheSrc& s, int64_t i) {
   for (int k = 0; k < 4; ++k) {
@@ -44,41 +56,38 @@ __device__ __forceinline__ unsigned int cache_src_word(const CacheSrc& s, int64_
   return 0;
 }
 
-// clears h->ok if any word differs from the copy
-__global__ void __launch_bounds__(256) cache_compare_kernel(CacheHdr* h, const CacheSrc src,
-                                                            const unsigned int* __restrict__ copy) {
-  if (h->ok == 0) return;
-  bool same = true;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < src.total;
-       i += (int64_t)gridDim.x * blockDim.x)
-    same = same && cache_src_word(src, i) == copy[i];
-  if (!__all(same) && (threadIdx.x & 63) == 0) h->ok = 0;
-}
-
-// after a rebuild: store the copy and the key (the next call compares against them)
-__global__ void __launch_bounds__(256) cache_commit_kernel(CacheHdr* h, const CacheSrc src,
-                                                           unsigned int* __restrict__ copy, unsigned long long key) {
-  if (h->ok) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < src.total;
-       i += (int64_t)gridDim.x * blockDim.x)
-    copy[i] = cache_src_word(src, i);
+// Compare the words with the copy and store them over it (see the header comment).
+__global__ void __launch_bounds__(256) cache_check_kernel(CacheHdr* h, const CacheSrc src,
+                                                          unsigned int* __restrict__ copy, unsigned long long key,
+                                                          unsigned long long call) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    h->key_ok = (h->valid == 1 && h->key == key) ? 1 : 0;
     h->key = key;
     h->valid = 1;
+    h->call = call;
   }
+  bool same = true;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < src.total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned int w = cache_src_word(src, i);
+    const unsigned int c = copy[i];
+    same = same & (w == c);
+    copy[i] = w;
+  }
+  if (!__all(same) && (threadIdx.x & 63) == 0) h->stale = call;
 }
 
-// a path that uses the workspace without the cache marks it stale
+// marks the header stale (a plan build that could not be enqueued completely)
 __global__ void cache_invalidate_kernel(CacheHdr* h) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     h->valid = 0;
-    h->ok = 0;
+    h->key_ok = 0;
   }
 }
 
 template <typename E>
 __global__ void __launch_bounds__(256) zero_fill_unless_kernel(E* __restrict__ p, int64_t n, const CacheHdr* h) {
-  if (h->ok) return;
+  if (cache_hit(h)) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = E(0);
 }
@@ -136,31 +145,19 @@ inline int64_t plan_cache_bytes(const tkbn_interp_desc* d) {
 // True when the caller asserted that the workspace's plan is current (no plan work).
 inline bool plan_reused(const tkbn_interp_desc* d) { return (d->flags & TKBN_PLAN_REUSE) != 0; }
 
-// Enqueues begin + compare; returns the header (device) for the plan kernels.
+// Enqueues the check kernel; returns the header (device) for the plan kernels.
 inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const int* tile, const void* omega, void* ws,
                             hipStream_t s, CacheHdr** hdr) {
   CacheHdr* h = static_cast<CacheHdr*>(ws);
   const CacheSrc src = plan_cache_src(d, omega);
-  const unsigned int* copy = reinterpret_cast<const unsigned int*>(static_cast<char*>(ws) + kCacheHdrBytes);
-  cache_begin_kernel<<<1, 64, 0, s>>>(h, plan_cache_key(d, kind, tile));
-  TKBN_LAUNCH_CHECK("cache_begin_kernel");
-  const int64_t blocks = std::min<int64_t>(ceil_div(src.total, 256), 1024);
-  if (blocks > 0) {
-    cache_compare_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(h, src, copy);
-    TKBN_LAUNCH_CHECK("cache_compare_kernel");
-  }
-  *hdr = h;
-  return TKBN_OK;
-}
-
-inline int plan_cache_commit(const tkbn_interp_desc* d, int kind, const int* tile, const void* omega, void* ws,
-                             hipStream_t s) {
-  CacheHdr* h = static_cast<CacheHdr*>(ws);
-  const CacheSrc src = plan_cache_src(d, omega);
   unsigned int* copy = reinterpret_cast<unsigned int*>(static_cast<char*>(ws) + kCacheHdrBytes);
+  static std::atomic<unsigned long long> calls{0};
+  const unsigned long long call = ++calls;  // from 1: a zeroed header never holds it
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(src.total, 256), 1024));
-  cache_commit_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(h, src, copy, plan_cache_key(d, kind, tile));
-  TKBN_LAUNCH_CHECK("cache_commit_kernel");
+  cache_check_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(h, src, copy, plan_cache_key(d, kind, tile),
+                                                                    call);
+  TKBN_LAUNCH_CHECK("cache_check_kernel");
+  *hdr = h;
   return TKBN_OK;
 }
 

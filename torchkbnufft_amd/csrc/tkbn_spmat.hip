@@ -26,30 +26,84 @@
 namespace tkbn {
 
 constexpr int kSpWaves = 4;  // waves per workgroup
+constexpr int kSpChunk = 64;  // workgroups dealt to one XCD in a row
 
-// seg: nseg x {row, first nonzero, end nonzero, partial slot or -1}
+// Workgroup b runs on XCD b mod 8; deal the logical blocks (consecutive rows) to the
+// XCDs in chunks of kSpChunk so that neighbouring rows, which read overlapping
+// columns, share an L2.  A bijection on [0, gridDim.x) when gridDim.x is a multiple
+// of 8 * kSpChunk.
+__device__ __forceinline__ int64_t spmv_block(int64_t b) {
+  const int64_t x = b % 8, k = b / 8;
+  return kSpChunk * (x + 8 * (k / kSpChunk)) + k % kSpChunk;
+}
+
+// value of lane j (j uniform over the wave when K == 1, per lane otherwise)
+template <int K>
+__device__ __forceinline__ int bcast(int v, int j) {
+  if constexpr (K == 1) return __builtin_amdgcn_readlane(v, j);
+  else return __shfl(v, j);
+}
+template <int K>
+__device__ __forceinline__ float bcast(float v, int j) {
+  return __int_as_float(bcast<K>(__float_as_int(v), j));
+}
+template <int K>
+__device__ __forceinline__ double bcast(double v, int j) {
+  const int lo = bcast<K>((int)(__double_as_longlong(v) & 0xffffffffll), j);
+  const int hi = bcast<K>((int)(__double_as_longlong(v) >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// seg: nseg x {row, first nonzero, end nonzero, partial slot or -1}.  A wave takes a
+// segment; lane (bl, k) covers batch element b0 + bl and nonzero slot k of K.  The
+// segment's columns and values are read 64 at a time by one coalesced load each and
+// handed to the lanes by cross-lane reads, so the x loads of a group of nonzeros
+// issue back to back.
 template <typename T, int LB>
 __global__ void __launch_bounds__(64 * kSpWaves)
-    spmv_seg_kernel(int64_t nseg, const int64_t* __restrict__ seg, const int64_t* __restrict__ col,
+    spmv_seg_kernel(int64_t nseg, const int64_t* __restrict__ seg, const int* __restrict__ col,
                     const cplx<T>* __restrict__ val, const cplx<T>* __restrict__ xt, int64_t batch,
                     cplx<T>* __restrict__ yt, cplx<T>* __restrict__ part) {
   constexpr int K = 64 / LB;  // nonzero slots per wave
+  constexpr int U = K == 1 ? 8 : 4;  // loads in flight per lane
   const int lane = threadIdx.x & 63;
   const int bl = lane % LB, k = lane / LB;
   const int64_t nwaves = (int64_t)gridDim.x * kSpWaves;
-  for (int64_t s = (int64_t)blockIdx.x * kSpWaves + (threadIdx.x >> 6); s < nseg; s += nwaves) {
+  for (int64_t s = spmv_block(blockIdx.x) * kSpWaves + (threadIdx.x >> 6); s < nseg; s += nwaves) {
     const int64_t row = seg[4 * s], e0 = seg[4 * s + 1], e1 = seg[4 * s + 2], slot = seg[4 * s + 3];
     cplx<T>* dst = slot < 0 ? yt + row * batch : part + slot * batch;
     for (int64_t b0 = 0; b0 < batch; b0 += LB) {
       const int64_t b = b0 + bl;
-      const int64_t bb = b < batch ? b : batch - 1;  // idle lanes read an in-range column
+      const cplx<T>* xb = xt + (b < batch ? b : batch - 1);  // idle lanes read an in-range element
       T ar = 0, ai = 0;
-#pragma unroll 4
-      for (int64_t e = e0 + k; e < e1; e += K) {
-        const cplx<T> v = val[e];
-        const cplx<T> xv = xt[col[e] * batch + bb];
-        ar += v.x * xv.x - v.y * xv.y;
-        ai += v.x * xv.y + v.y * xv.x;
+      for (int64_t c0 = e0; c0 < e1; c0 += 64) {
+        const int n = (int)(e1 - c0 < 64 ? e1 - c0 : 64);
+        const int64_t ei = c0 + (lane < n ? lane : 0);
+        const int ci = col[ei];
+        const T vr_i = val[ei].x, vi_i = val[ei].y;
+        // steps of U * K nonzeros; slots past n read column ci of lane >= n (a valid
+        // column) with weight 0
+        for (int j = 0; j < n; j += U * K) {
+          cplx<T> xv[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int jj = j + u * K + k;
+            const int c = bcast<K>(ci, jj < 64 ? jj : 63);
+            xv[u] = xb[(int64_t)c * batch];
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int jj = j + u * K + k;
+            const int jc = jj < 64 ? jj : 63;
+            // every lane takes part in the cross-lane reads (a lane outside a branch
+            // would hand out no value), then slots past n drop theirs
+            const T vr0 = bcast<K>(vr_i, jc), vi0 = bcast<K>(vi_i, jc);
+            const bool on = jj < n;
+            const T vr = on ? vr0 : T(0), vi = on ? vi0 : T(0);
+            ar += vr * xv[u].x - vi * xv[u].y;
+            ai += vr * xv[u].y + vi * xv[u].x;
+          }
+        }
       }
 #pragma unroll
       for (int off = LB; off < 64; off <<= 1) {
@@ -126,7 +180,7 @@ void launch_transpose(int64_t rows, int64_t cols, const void* src, void* dst, hi
 
 template <typename T>
 int run_spmv(int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const int64_t* seg, int64_t nfix,
-             const int64_t* fix, int64_t nslots, const int64_t* col, const void* val, const void* x, void* y,
+             const int64_t* fix, int64_t nslots, const int* col, const void* val, const void* x, void* y,
              char* ws, hipStream_t s) {
   const SpmvWs w = spmv_ws_layout(sizeof(cplx<T>), nrows, ncols, batch, nslots);
   const bool tr = batch > 1;
@@ -140,7 +194,10 @@ int run_spmv(int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const in
   }
   cplx<T>* part = reinterpret_cast<cplx<T>*>(ws + w.part);
   if (nseg > 0) {
-    const unsigned blocks = (unsigned)(ceil_div(nseg, kSpWaves) < (1 << 20) ? ceil_div(nseg, kSpWaves) : (1 << 20));
+    // a multiple of 8 * kSpChunk (see spmv_block); surplus waves find no segment
+    int64_t nb = ceil_div(nseg, kSpWaves);
+    nb = nb < (1 << 20) ? nb : (1 << 20);
+    const unsigned blocks = (unsigned)(ceil_div(nb, 8 * kSpChunk) * 8 * kSpChunk);
     const auto* v = static_cast<const cplx<T>*>(val);
 #define TKBN_SPMV(LBV) spmv_seg_kernel<T, LBV><<<blocks, dim3(64 * kSpWaves), 0, s>>>(nseg, seg, col, v, xt, batch, yt, part)
     if (batch == 1)
@@ -185,13 +242,14 @@ int64_t tkbn_spmv_workspace_bytes(int32_t dtype, int64_t nrows, int64_t ncols, i
 }
 
 int tkbn_spmv(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const int64_t* seg,
-              int64_t nfix, const int64_t* fix, int64_t nslots, const int64_t* col, const void* val, const void* x,
+              int64_t nfix, const int64_t* fix, int64_t nslots, const int32_t* col, const void* val, const void* x,
               void* y, void* workspace, int64_t workspace_bytes, void* stream) {
   TKBN_REQUIRE(dtype == TKBN_F32 || dtype == TKBN_F64, TKBN_ERR_INVALID, "dtype must be TKBN_F32 or TKBN_F64");
   TKBN_REQUIRE(nrows >= 0 && ncols >= 0 && batch >= 0 && nseg >= 0 && nfix >= 0 && nslots >= 0, TKBN_ERR_INVALID,
                "negative size");
   if (nrows == 0 || batch == 0) return TKBN_OK;
   TKBN_REQUIRE(nseg >= nrows, TKBN_ERR_INVALID, "every row needs at least one segment");
+  TKBN_REQUIRE(ncols <= INT32_MAX, TKBN_ERR_UNSUPPORTED, "more than 2^31 - 1 columns");
   TKBN_REQUIRE(seg && y && (nfix == 0 || fix) && x, TKBN_ERR_INVALID, "NULL pointer");
   const int64_t need = tkbn_spmv_workspace_bytes(dtype, nrows, ncols, batch, nslots);
   TKBN_REQUIRE(need == 0 || (workspace && workspace_bytes >= need), TKBN_ERR_INVALID,

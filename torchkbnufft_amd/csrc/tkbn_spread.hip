@@ -182,7 +182,7 @@ template <typename T, int D>
 __global__ void __launch_bounds__(256)
     bin_count_kernel(const Geom<T> g, const TileGeom tg, const T* __restrict__ omega, int* __restrict__ ws,
                      const CacheHdr* cache) {
-  if (cache->ok) return;  // cached plan reused
+  if (cache_hit(cache)) return;  // cached plan reused
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   const int64_t b = blockIdx.z;
@@ -197,7 +197,7 @@ template <typename T, int D>
 __global__ void __launch_bounds__(256)
     bin_scatter_kernel(const Geom<T> g, const TileGeom tg, const T* __restrict__ omega, int* __restrict__ ws,
                        const CacheHdr* cache) {
-  if (cache->ok) return;  // cached plan reused
+  if (cache_hit(cache)) return;  // cached plan reused
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   const int64_t b = blockIdx.z;
@@ -239,7 +239,7 @@ __device__ void block_exclusive_scan(const int* in, int* out, int n, int* sbuf) 
 
 __global__ void __launch_bounds__(1024) plan_scan_kernel(const TileGeom tg, int* __restrict__ ws,
                                                         const CacheHdr* cache = nullptr) {
-  if (cache && cache->ok) return;  // cached plan reused
+  if (cache && cache_hit(cache)) return;  // cached plan reused
   __shared__ int sbuf[1024];
   int* plan = ws + (int64_t)blockIdx.x * tg.plan_ints;
   const int* count = plan + tg.o_count;
@@ -618,7 +618,7 @@ __device__ __forceinline__ void coil_sample_tiles(const Geom<float>& g, const Ti
 __global__ void __launch_bounds__(256)
     coil_bin_count_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
                           int* __restrict__ ws, const CacheHdr* cache) {
-  if (cache->ok) return;  // cached plan reused
+  if (cache_hit(cache)) return;  // cached plan reused
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   int tl[2][3], off[2][3], n[2];
@@ -631,7 +631,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     coil_bin_scatter_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
                             int* __restrict__ ws, const CacheHdr* cache) {
-  if (cache->ok) return;  // cached plan reused
+  if (cache_hit(cache)) return;  // cached plan reused
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   int tl[2][3], off[2][3], n[2];
@@ -675,7 +675,7 @@ __global__ void __launch_bounds__(256)
 // each tile's keys are offset locally.  The totals also give the splits and work list.
 __global__ void __launch_bounds__(1024) coil_plan_scan_kernel(const TileGeom tg, int* __restrict__ ws,
                                                              const CacheHdr* cache) {
-  if (cache->ok) return;  // cached plan reused
+  if (cache_hit(cache)) return;  // cached plan reused
   __shared__ int sbuf[1024];
   int* plan = ws;
   const int* count = plan + tg.o_count;
@@ -968,7 +968,7 @@ __device__ __forceinline__ int gat_base_tile(const Geom<float>& g, const TileGeo
 __global__ void __launch_bounds__(256)
     gat_bin_count_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega, int* __restrict__ ws,
                          const CacheHdr* cache) {
-  if (cache->ok) return;  // cached plan reused
+  if (cache_hit(cache)) return;  // cached plan reused
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   int l[2];
@@ -980,7 +980,7 @@ __global__ void __launch_bounds__(256)
     gat_bin_scatter_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
                            int* __restrict__ ws,
                          const CacheHdr* cache) {
-  if (cache->ok) return;  // cached plan reused
+  if (cache_hit(cache)) return;  // cached plan reused
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = m < g.M;
   int l[2] = {0, 0};
@@ -1010,7 +1010,7 @@ __global__ void __launch_bounds__(256)
 // pieces of `target`, listed first (longest work first); the rest follow, one each.
 __global__ void __launch_bounds__(1024) gat_plan_scan_kernel(const TileGeom tg, int* __restrict__ ws,
                                                             const CacheHdr* cache) {
-  if (cache->ok) return;  // cached plan reused
+  if (cache_hit(cache)) return;  // cached plan reused
   __shared__ int sbuf[1024];
   int* plan = ws;
   const int* count = plan + tg.o_count;
@@ -1569,6 +1569,25 @@ static void profile_after(hipStream_t s) {
   g_profile = ProfileHook{};
 }
 
+// count -> scan -> scatter of the multi-coil spread plan (each exits at once on a cache hit)
+static int coil_spread_plan(const Geom<float>& g, const TileGeom& tg, const void* omega, int* wsi,
+                            const CacheHdr* cache, hipStream_t s) {
+  int st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
+  if (st != TKBN_OK) return st;
+  dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
+  if (g.M > 0) {
+    coil_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
+    TKBN_LAUNCH_CHECK("coil_bin_count_kernel");
+  }
+  coil_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
+  TKBN_LAUNCH_CHECK("coil_plan_scan_kernel");
+  if (g.M > 0) {
+    coil_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
+    TKBN_LAUNCH_CHECK("coil_bin_scatter_kernel");
+  }
+  return TKBN_OK;
+}
+
 static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, const void* omega, void* ws,
                                void* grid, hipStream_t s) {
   const Geom<float> g = make_geom<float>(d);
@@ -1579,21 +1598,11 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
     CacheHdr* cache = nullptr;
     st = plan_cache_begin(d, kPlanSpreadCoils, tg.T, omega, ws, s, &cache);
     if (st != TKBN_OK) return st;
-    st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
-    if (st != TKBN_OK) return st;
-    dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
-    if (g.M > 0) {
-      coil_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-      TKBN_LAUNCH_CHECK("coil_bin_count_kernel");
+    st = coil_spread_plan(g, tg, omega, wsi, cache, s);
+    if (st != TKBN_OK) {
+      plan_cache_invalidate(ws, s);
+      return st;
     }
-    coil_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
-    TKBN_LAUNCH_CHECK("coil_plan_scan_kernel");
-    if (g.M > 0) {
-      coil_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-      TKBN_LAUNCH_CHECK("coil_bin_scatter_kernel");
-    }
-    st = plan_cache_commit(d, kPlanSpreadCoils, tg.T, omega, ws, s);
-    if (st != TKBN_OK) return st;
   }
   const int64_t nrows = g.B * g.C;
   const unsigned ychunks = (unsigned)ceil_div(nrows, 64);
@@ -1642,11 +1651,14 @@ int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const v
   if (!reuse) {
     // counts must start at zero: clear the plan area unless the cached plan is reused
     st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
-    if (st != TKBN_OK) return st;
+    if (st != TKBN_OK) {
+      plan_cache_invalidate(ws, s);
+      return st;
+    }
   }
   st = spread_dispatch<T>(g, tg, det, kdata, omega, wsi, slot, grid, lds, s, cache, d->ndim, max_numpoints(d));
-  if (st != TKBN_OK) return st;
-  return reuse ? TKBN_OK : plan_cache_commit(d, kind, tg.T, omega, ws, s);
+  if (st != TKBN_OK && !reuse) plan_cache_invalidate(ws, s);
+  return st;
 }
 
 template <typename T, int D, int JM>
@@ -1691,11 +1703,14 @@ int interp_fwd_tiled_t(const tkbn_interp_desc* d, const void* grid, const void* 
     st = plan_cache_begin(d, kPlanFwdTiles, tg.T, omega, ws, s, &cache);
     if (st != TKBN_OK) return st;
     st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
-    if (st != TKBN_OK) return st;
+    if (st != TKBN_OK) {
+      plan_cache_invalidate(ws, s);
+      return st;
+    }
   }
   st = fwd_tiled_dispatch<T>(g, tg, grid, omega, wsi, kdata, s, cache, d->ndim, max_numpoints(d));
-  if (st != TKBN_OK) return st;
-  return reuse ? TKBN_OK : plan_cache_commit(d, kPlanFwdTiles, tg.T, omega, ws, s);
+  if (st != TKBN_OK && !reuse) plan_cache_invalidate(ws, s);
+  return st;
 }
 
 // ---- multi-coil forward gather ---------------------------------------------------
@@ -1746,6 +1761,21 @@ static int64_t gather_workspace_bytes(const TileGeom& tg) {
   return tg.o_rec_bytes + tg.maxent * (int64_t)sizeof(GatRec);
 }
 
+// count -> scan -> scatter of the multi-coil gather plan (each exits at once on a cache hit)
+static int gather_coils_plan(const Geom<float>& g, const TileGeom& tg, const void* omega, int* wsi,
+                             const CacheHdr* cache, hipStream_t s) {
+  int st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
+  if (st != TKBN_OK) return st;
+  dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
+  gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
+  TKBN_LAUNCH_CHECK("gat_bin_count_kernel");
+  gat_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
+  TKBN_LAUNCH_CHECK("gat_plan_scan_kernel");
+  gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
+  TKBN_LAUNCH_CHECK("gat_bin_scatter_kernel");
+  return TKBN_OK;
+}
+
 static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, const void* omega, void* kdata, void* ws,
                                hipStream_t s) {
   const Geom<float> g = make_geom<float>(d);
@@ -1756,17 +1786,11 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
     CacheHdr* cache = nullptr;
     st = plan_cache_begin(d, kPlanGatherCoils, tg.T, omega, ws, s, &cache);
     if (st != TKBN_OK) return st;
-    st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
-    if (st != TKBN_OK) return st;
-    dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
-    gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-    TKBN_LAUNCH_CHECK("gat_bin_count_kernel");
-    gat_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
-    TKBN_LAUNCH_CHECK("gat_plan_scan_kernel");
-    gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-    TKBN_LAUNCH_CHECK("gat_bin_scatter_kernel");
-    st = plan_cache_commit(d, kPlanGatherCoils, tg.T, omega, ws, s);
-    if (st != TKBN_OK) return st;
+    st = gather_coils_plan(g, tg, omega, wsi, cache, s);
+    if (st != TKBN_OK) {
+      plan_cache_invalidate(ws, s);
+      return st;
+    }
   }
   const int nchunk = (int)ceil_div(g.B * g.C, kGatRows);
   const int64_t groups = ceil_div(tg.maxwork * nchunk, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
