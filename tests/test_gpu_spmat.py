@@ -235,3 +235,15 @@ def test_spmv_user_matrices(shared):
     x = tkbn.functional.kb_spmat_interp_adjoint(data.to(DEV), mats, torch.tensor([12, 10]))
     assert rel_l2(y.reshape(6, -1), torch.from_numpy(grid.reshape(6, -1).numpy() @ A.T)) < 1e-13
     assert rel_l2(x.reshape(6, -1), torch.from_numpy(data.reshape(6, -1).numpy() @ A.conj())) < 1e-13
+
+
+def test_spmat_without_samples():
+    """An empty trajectory: empty k-space forward, zero grid adjoint."""
+    im = (8, 6)
+    om = torch.zeros(2, 0, device=DEV)
+    mats = tkbn.calc_tensor_spmatrix(om, im)
+    grid = torch.ones(2, 3, 16, 12, dtype=torch.complex64, device=DEV)
+    y = tkbn.KbInterp(im_size=im, device=DEV)(grid, om, mats)
+    assert y.shape == (2, 3, 0)
+    x = tkbn.KbInterpAdjoint(im_size=im, device=DEV)(torch.zeros(2, 3, 0, dtype=torch.complex64, device=DEV), om, mats)
+    assert x.shape == (2, 3, 16, 12) and not x.abs().any()

@@ -135,3 +135,10 @@ def test_plan_merges_real_and_imaginary_parts(adjoint, shared):
     assert (plan.nrows, plan.ncols) == dense.shape
     x = rng.standard_normal((2, dense.shape[1])) + 1j * rng.standard_normal((2, dense.shape[1]))
     np.testing.assert_allclose(_emulate(plan, x), x @ dense.T, rtol=1e-12, atol=1e-12)
+
+
+def test_spmatrix_without_samples():
+    re, im = tkbn.calc_tensor_spmatrix(torch.zeros(2, 0), (8, 6))
+    assert re.shape == (0, 16 * 12) and re._nnz() == 0 and im._nnz() == 0
+    re, im = tkbn.calc_tensor_spmatrix(torch.zeros(3, 0, dtype=torch.float64), (4, 4, 4))
+    assert re.shape == (0, 512) and re.dtype == torch.float64

@@ -74,8 +74,9 @@ def _coo_entries(
         if coef is None:
             coef, cols = c_k, col_k
         else:
-            coef = (coef[:, None, :] * c_k[:, :, None]).reshape(m, -1)
-            cols = (cols[:, None, :] + col_k[:, :, None]).reshape(m, -1)
+            taps = coef.shape[1] * J
+            coef = (coef[:, None, :] * c_k[:, :, None]).reshape(m, taps)
+            cols = (cols[:, None, :] + col_k[:, :, None]).reshape(m, taps)
     shift = np.exp(1j * np.dot(np.transpose(omega), np.expand_dims(np.asarray(n_shift), 1)))
     vals = np.conj(coef) * shift
     rows = np.repeat(np.arange(m, dtype=np.int64), coef.shape[1])
@@ -245,7 +246,7 @@ def spmat_interp(image: Tensor, interp_mats: Tuple[Tensor, Tensor]) -> Tensor:
         raise TypeError(f"image must be complex64 or complex128, got {image.dtype}")
     b, c = image.shape[:2]
     plan = _plan(interp_mats, False, image)
-    x = image.reshape(b * c, -1)
+    x = image.reshape(b * c, math.prod(image.shape[2:]))
     if x.shape[1] != plan.ncols:
         raise RuntimeError(
             f"interpolation matrix has {plan.ncols} grid columns but the grid has {x.shape[1]} points"
@@ -263,7 +264,7 @@ def spmat_interp_adjoint(data: Tensor, interp_mats: Tuple[Tensor, Tensor], grid_
     gs = [int(v) for v in host_values(grid_size)]
     b, c = data.shape[:2]
     plan = _plan(interp_mats, True, data)
-    x = data.reshape(b * c, -1)
+    x = data.reshape(b * c, data.shape[-1])
     if x.shape[1] != plan.ncols:
         raise RuntimeError(f"interpolation matrix has {plan.ncols} samples but the data has {x.shape[1]}")
     if math.prod(gs) != plan.nrows:

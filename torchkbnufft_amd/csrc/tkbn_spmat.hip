@@ -173,6 +173,7 @@ inline SpmvWs spmv_ws_layout(size_t esize, int64_t nrows, int64_t ncols, int64_t
 
 template <typename T>
 void launch_transpose(int64_t rows, int64_t cols, const void* src, void* dst, hipStream_t s) {
+  if (rows == 0 || cols == 0) return;
   const dim3 grid((unsigned)(ceil_div(rows, 32) * ceil_div(cols, 32)));
   transpose_kernel<T><<<grid, dim3(256), 0, s>>>(rows, cols, static_cast<const cplx<T>*>(src),
                                                   static_cast<cplx<T>*>(dst));
@@ -250,7 +251,7 @@ int tkbn_spmv(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch, int64_
   if (nrows == 0 || batch == 0) return TKBN_OK;
   TKBN_REQUIRE(nseg >= nrows, TKBN_ERR_INVALID, "every row needs at least one segment");
   TKBN_REQUIRE(ncols <= INT32_MAX, TKBN_ERR_UNSUPPORTED, "more than 2^31 - 1 columns");
-  TKBN_REQUIRE(seg && y && (nfix == 0 || fix) && x, TKBN_ERR_INVALID, "NULL pointer");
+  TKBN_REQUIRE(seg && y && (nfix == 0 || fix) && (ncols == 0 || x), TKBN_ERR_INVALID, "NULL pointer");
   const int64_t need = tkbn_spmv_workspace_bytes(dtype, nrows, ncols, batch, nslots);
   TKBN_REQUIRE(need == 0 || (workspace && workspace_bytes >= need), TKBN_ERR_INVALID,
                "workspace of %lld bytes needed", (long long)need);
