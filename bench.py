@@ -332,7 +332,7 @@ def bench_toeplitz(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
     }
 
 
-def bench_spmat(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
+def bench_spmat(dev, x, ktraj, smaps, iters: int, pair_ms: float, with_cpu: bool = False) -> dict:
     """The SENSE pair with sparse-matrix interpolation (calc_tensor_spmatrix matrices,
     exact Kaiser-Bessel weights) instead of the table.  Algorithmic bytes of one
     interpolation: the matrix once (4-byte column + 8-byte complex value per nonzero),
@@ -369,6 +369,21 @@ def bench_spmat(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
     }
     nnz = mats[0]._nnz()
     alg = nnz * 12 + B * C * (K * K + M) * 8
+    cpu = {}
+    if with_cpu:
+        # the oracle's restatement of the reference's four torch.mm products, one
+        # forward and one adjoint product on the same 64 rows, host threads
+        from oracle import table_path as tp
+
+        cm = tuple(m.cpu() for m in mats)
+        gc, dc = grid.cpu(), data.cpu()
+        t0 = time.perf_counter()
+        tp.spmat_interp(gc, cm)
+        cpu["cpu_oracle_fwd_ms"] = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        tp.spmat_interp_adjoint(dc, cm, (K, K))
+        cpu["cpu_oracle_adj_ms"] = (time.perf_counter() - t0) * 1e3
+        cpu["cpu_threads"] = torch.get_num_threads()
     return {
         "workload": f"KbNufft + KbNufftAdjoint with interp_mats on the same batch ({B} x {C} coils, {N}x{N} -> "
         f"{K}x{K}, {M} radial samples, {nnz} nonzeros)",
@@ -382,6 +397,7 @@ def bench_spmat(dev, x, ktraj, smaps, iters: int, pair_ms: float) -> dict:
         "interp_fwd_hbm_frac": alg / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "interp_adj_hbm_frac": alg / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "one_coil_interp": one,
+        **cpu,
     }
 
 
@@ -549,7 +565,8 @@ def main() -> int:
     fw = tkbn.KbNufft(im_size=(N, N), numpoints=J, table_oversamp=L, dtype=torch.complex64).to(dev)
     ad = tkbn.KbNufftAdjoint(im_size=(N, N), numpoints=J, table_oversamp=L, dtype=torch.complex64).to(dev)
     if args.workload == "spmat":
-        print(json.dumps(bench_spmat(dev, x, ktraj, smaps, args.form_iters, None)), flush=True)
+        print(json.dumps(bench_spmat(dev, x, ktraj, smaps, args.form_iters, None, not args.no_cpu_baseline)),
+              flush=True)
         return 0
 
     def step():
@@ -624,7 +641,7 @@ def main() -> int:
         out["dcomp"] = bench_dcomp(dev, ktraj, args.form_iters, not args.no_cpu_baseline)
         out["toeplitz"] = bench_toeplitz(dev, x, ktraj, smaps, max(args.form_iters, 10), ms)
         out["small"] = bench_small(dev, max(args.form_iters, 20), not args.no_cpu_baseline)
-        out["spmat"] = bench_spmat(dev, x, ktraj, smaps, args.form_iters, ms)
+        out["spmat"] = bench_spmat(dev, x, ktraj, smaps, args.form_iters, ms, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, ktraj, smaps, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

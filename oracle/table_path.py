@@ -15,6 +15,9 @@ complex64 results reproduce the reference's float32 index arithmetic:
 * forward ``y = sum_o coef * X[idx]`` then ``y *= exp(i * sum_d omega_d * n_shift_d)``
   (``interp.py:185-203``); adjoint is the conjugate transpose with the
   reference's sort-by-``tm`` accumulation order (``interp.py:552-584,680-724``).
+
+Also restated: density compensation (``dcomp.py``), the Toeplitz kernel and filter
+(``toep.py``, ``fft.py:121-173``) and the sparse-matrix products (``interp.py:13-85``).
 """
 from __future__ import annotations
 
@@ -376,6 +379,32 @@ def toep_nufft(image, kernel, smaps=None, norm=None):
     return torch.stack(out)
 
 
+# ----------------------------------------------------------------------------
+# sparse-matrix interpolation (interp.py:13-85): four real products per direction
+# ----------------------------------------------------------------------------
+def spmat_interp(image, interp_mats):
+    """Grid (B, C, *K) -> samples (B, C, M) with the (real, imaginary) matrix pair
+    (``interp.py:13-47``: ``(R x_re - I x_im) + i (R x_im + I x_re)``)."""
+    re, im = interp_mats
+    b, c = image.shape[:2]
+    x = torch.view_as_real(image).reshape(b * c, -1, 2)
+    xr, xi = x[..., 0].t().contiguous(), x[..., 1].t().contiguous()
+    out = torch.stack([(torch.mm(re, xr) - torch.mm(im, xi)).t(), (torch.mm(re, xi) + torch.mm(im, xr)).t()], dim=-1)
+    return torch.view_as_complex(out.contiguous()).reshape(b, c, -1)
+
+
+def spmat_interp_adjoint(data, interp_mats, grid_size):
+    """Samples (B, C, M) -> grid (B, C, *grid_size) with the transposed pair and the
+    conjugate (``interp.py:50-85``: ``(R^T y_re + I^T y_im) + i (R^T y_im - I^T y_re)``)."""
+    re, im = interp_mats
+    b, c = data.shape[:2]
+    y = torch.view_as_real(data).reshape(b * c, -1, 2)
+    yr, yi = y[..., 0].t().contiguous(), y[..., 1].t().contiguous()
+    rt, it = re.t(), im.t()
+    out = torch.stack([(torch.mm(rt, yr) + torch.mm(it, yi)).t(), (torch.mm(rt, yi) - torch.mm(it, yr)).t()], dim=-1)
+    return torch.view_as_complex(out.contiguous()).reshape([b, c] + [int(v) for v in grid_size])
+
+
 def inner_product(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
     """``<a, b> = sum(conj(b) * a)`` (``_math.py:138-166``)."""
     return torch.sum(b.conj() * a)
@@ -401,6 +430,8 @@ __all__: List[str] = [
     "kb_table_nufft_adjoint",
     "kb_table_interp",
     "kb_table_interp_adjoint",
+    "spmat_interp",
+    "spmat_interp_adjoint",
     "inner_product",
     "rel_l2",
 ]

@@ -142,3 +142,20 @@ def test_spmatrix_without_samples():
     assert re.shape == (0, 16 * 12) and re._nnz() == 0 and im._nnz() == 0
     re, im = tkbn.calc_tensor_spmatrix(torch.zeros(3, 0, dtype=torch.float64), (4, 4, 4))
     assert re.shape == (0, 512) and re.dtype == torch.float64
+
+
+def test_oracle_spmat_products_match_dense():
+    """The oracle's restatement of the reference's four-product spmat_interp pair
+    against the dense matrix (pins it for the CPU baseline)."""
+    from oracle import table_path as tp
+
+    gen = torch.Generator().manual_seed(12)
+    om = gi.uniform_ktraj(2, 40, gen)
+    mats = tkbn.calc_tensor_spmatrix(om, (6, 5))
+    A = _dense(mats)
+    grid = torch.complex(torch.randn(2, 3, 12, 10, dtype=torch.float64), torch.randn(2, 3, 12, 10, dtype=torch.float64))
+    data = torch.complex(torch.randn(2, 3, 40, dtype=torch.float64), torch.randn(2, 3, 40, dtype=torch.float64))
+    y = tp.spmat_interp(grid, mats)
+    x = tp.spmat_interp_adjoint(data, mats, (12, 10))
+    np.testing.assert_allclose(y.reshape(6, -1).numpy(), grid.reshape(6, -1).numpy() @ A.T, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(x.reshape(6, -1).numpy(), data.reshape(6, -1).numpy() @ A.conj(), rtol=1e-12, atol=1e-12)
