@@ -211,24 +211,27 @@ TKBN_API int tkbn_fft_filter(int32_t dtype, int32_t ndim, const int64_t* im_size
 
 /*
  * Sparse-matrix interpolation, replacing the torch.mm products of
- * _nufft/interp.py:13-85 (spmat_interp, spmat_interp_adjoint).  A is an nrows x ncols
- * complex CSR matrix (col: int32 column of each nonzero, val: complex value, rows
- * stored in order); x is (batch, ncols) and y is (batch, nrows), both contiguous:
- *   y[b, r] = sum over the nonzeros e of row r of val[e] * x[b, col[e]]
- * The rows are given as segments, seg = nseg x {row, first nonzero, end nonzero, slot}
- * (int64, row-major): every row's nonzeros are covered by its segments in order.  A
- * row with one segment has slot -1; the segments of a row with several take distinct
- * slots in [0, nslots), and fix = nfix x {row, first slot, slot count} lists those
- * rows, their slots consecutive in segment order.  Sums run in a fixed order (no
- * atomics).  Needs tkbn_spmv_workspace_bytes of workspace (0 when batch == 1 and
- * nslots == 0).  Columns must lie in [0, ncols), ncols < 2^31: they are not checked
- * on the device.
+ * _nufft/interp.py:13-85 (spmat_interp, spmat_interp_adjoint).  x is (batch, ncols)
+ * and y is (batch, nrows), complex, contiguous:  y[b, r] = sum_c A[r, c] * x[b, c].
+ * A is given in row blocks of R = tkbn_spmv_block_rows() consecutive rows: block q
+ * (rows q*R .. q*R+R-1) owns a run of union columns ucol[u] (int32, in [0, ncols),
+ * ncols < 2^31) and, per union column u, the R values uval[u*R + i] = A[q*R + i,
+ * ucol[u]] (complex, zero where the row has no entry).  The runs are given as
+ * segments, seg = nseg x {block, first union column, end union column, slot} (int64,
+ * row-major): every block's run is covered by its segments in order (an empty block
+ * has one empty segment).  A block with one segment has slot -1; the segments of a
+ * block with several take distinct slots in [0, nslots), and fix = nfix x {block,
+ * first slot, slot count} lists those blocks, their slots consecutive in segment
+ * order.  Sums run in a fixed order (no atomics).  Needs tkbn_spmv_workspace_bytes of
+ * workspace (0 when batch == 1 and nslots == 0).  Columns are not checked on the
+ * device.
  */
+TKBN_API int tkbn_spmv_block_rows(void);
 TKBN_API int64_t tkbn_spmv_workspace_bytes(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch,
                                            int64_t nslots);
 TKBN_API int tkbn_spmv(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg,
-                       const int64_t* seg, int64_t nfix, const int64_t* fix, int64_t nslots, const int32_t* col,
-                       const void* val, const void* x, void* y, void* workspace, int64_t workspace_bytes,
+                       const int64_t* seg, int64_t nfix, const int64_t* fix, int64_t nslots, const int32_t* ucol,
+                       const void* uval, const void* x, void* y, void* workspace, int64_t workspace_bytes,
                        void* stream);
 
 #ifdef __cplusplus

@@ -66,26 +66,28 @@ def test_spmatrix_errors():
 
 def _emulate(plan, x):
     """The sums tkbn_spmv performs, from the plan's arrays (x: (batch, ncols))."""
+    R = sp._BLOCK_ROWS
     seg, fix = plan.seg.numpy(), plan.fix.numpy()
-    col, val = plan.col.numpy(), plan.val.numpy()
-    yt = np.full((plan.nrows, x.shape[0]), np.nan, dtype=np.complex128)
-    part = np.full((plan.nslots, x.shape[0]), np.nan, dtype=np.complex128)
-    for row, e0, e1, slot in seg:
-        s = (val[e0:e1, None] * x[:, col[e0:e1]].T).sum(0)
+    ucol, uval = plan.ucol.numpy(), plan.uval.numpy()
+    nblk = -(-plan.nrows // R)
+    yt = np.full((nblk * R, x.shape[0]), np.nan, dtype=np.complex128)
+    part = np.full((plan.nslots * R, x.shape[0]), np.nan, dtype=np.complex128)
+    for blk, u0, u1, slot in seg:
+        s = uval[u0:u1].T @ x[:, ucol[u0:u1]].T  # (R, batch)
         if slot < 0:
-            yt[row] = s
+            yt[blk * R : blk * R + R] = s
         else:
-            part[slot] = s
-    for row, first, cnt in fix:
-        yt[row] = part[first : first + cnt].sum(0)
-    return yt.T
+            part[slot * R : slot * R + R] = s
+    for blk, first, cnt in fix:
+        yt[blk * R : blk * R + R] = part[first * R : (first + cnt) * R].reshape(cnt, R, -1).sum(0)
+    return yt[: plan.nrows].T
 
 
 def test_segment_plan_covers_rows_and_reproduces_product():
     rng = np.random.default_rng(3)
-    nrows, ncols = 40, 5000
+    nrows, ncols = 43, 5000
     counts = rng.integers(0, 12, nrows)
-    counts[[3, 17]] = [700, 300]  # rows split into segments; one empty row below
+    counts[[3, 17]] = [700, 300]  # blocks split into segments; one empty row below
     counts[25] = 0
     rows = np.repeat(np.arange(nrows), counts)
     cols = rng.integers(0, ncols, rows.size)  # duplicates included
@@ -93,16 +95,21 @@ def test_segment_plan_covers_rows_and_reproduces_product():
     perm = rng.permutation(rows.size)
     rows, cols, vals = rows[perm], cols[perm], vals[perm]
     plan = sp.SpmvPlan(torch.tensor(rows), torch.tensor(cols), torch.tensor(vals), nrows, ncols)
+    R = sp._BLOCK_ROWS
+    nblk = -(-nrows // R)
     seg = plan.seg.numpy()
-    # segments: row-ordered, contiguous per row, at most _SEG_NNZ nonzeros, covering crow
-    assert np.all(np.diff(seg[:, 0]) >= 0) and set(seg[:, 0]) == set(range(nrows))
-    assert np.all(seg[:, 2] - seg[:, 1] <= sp._SEG_NNZ) and np.all(seg[:, 2] >= seg[:, 1])
-    for r in range(nrows):
-        s = seg[seg[:, 0] == r]
+    # segments: block-ordered, contiguous per block, at most _SEG_COLS union columns
+    assert np.all(np.diff(seg[:, 0]) >= 0) and set(seg[:, 0]) == set(range(nblk))
+    assert np.all(seg[:, 2] - seg[:, 1] <= sp._SEG_COLS) and np.all(seg[:, 2] >= seg[:, 1])
+    for q in range(nblk):
+        s = seg[seg[:, 0] == q]
         assert np.all(s[1:, 1] == s[:-1, 2])
         assert (s[:, 3] < 0).all() == (len(s) == 1)
-    uniq = np.array([np.unique(cols[rows == r]).size for r in range(nrows)])
-    nseg = np.maximum(1, -(-uniq // sp._SEG_NNZ))
+        # the block's union columns: sorted, distinct, exactly its rows' columns
+        sel = (rows // R) == q
+        np.testing.assert_array_equal(plan.ucol.numpy()[s[0, 1] : s[-1, 2]], np.unique(cols[sel]))
+    union = np.array([np.unique(cols[(rows // R) == q]).size for q in range(nblk)])
+    nseg = np.maximum(1, -(-union // sp._SEG_COLS))
     assert plan.nseg == nseg.sum() and plan.nfix == (nseg > 1).sum() >= 1
     assert plan.nslots == nseg[nseg > 1].sum()
     dense = np.zeros((nrows, ncols), dtype=np.complex128)

@@ -88,6 +88,7 @@ _SIGNATURES = {
         [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64, _c_i64,
          _c_vp, _c_vp, _c_vp, _c_i32, _c_dbl, _c_dbl, _c_vp, _c_vp, _c_i64, _c_vp],
     ),
+    "tkbn_spmv_block_rows": (ctypes.c_int, []),
     "tkbn_spmv_workspace_bytes": (_c_i64, [_c_i32, _c_i64, _c_i64, _c_i64, _c_i64]),
     "tkbn_spmv": (
         ctypes.c_int,

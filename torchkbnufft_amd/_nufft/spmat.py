@@ -12,9 +12,11 @@ matches the reference's entry for entry.
 
 ``spmat_interp`` / ``spmat_interp_adjoint`` replace the reference's ``torch.mm``
 products (``_nufft/interp.py:13-85``) with the HIP kernel ``tkbn_spmv``.  The two real
-matrices are merged once into one complex CSR matrix A = A_re + i A_im (and its
-conjugate transpose for the adjoint), cut into row segments of at most
-``_SEG_NNZ`` nonzeros for load balance, and cached per matrix pair.
+matrices are merged once into one complex matrix A = A_re + i A_im (and its conjugate
+transpose for the adjoint), stored in blocks of ``_BLOCK_ROWS`` rows that share their
+union of columns (neighbouring rows of an interpolation matrix share most columns, so
+each input value is read once per block), cut into segments for load balance, and
+cached per matrix pair.
 """
 from __future__ import annotations
 
@@ -31,7 +33,8 @@ from .. import _lib
 from ._host import host_values, require_gpu
 from .utils import validate_args
 
-_SEG_NNZ = 256  # nonzeros per row segment (one wave each)
+_BLOCK_ROWS = 8  # rows per block of the sparse format (the library's tkbn_spmv_block_rows)
+_SEG_COLS = 128  # union columns per segment (one wave each)
 
 
 def _coo_entries(
@@ -118,17 +121,27 @@ def calc_tensor_spmatrix(
 
 
 class SpmvPlan:
-    """A complex CSR matrix cut into row segments, ready for ``tkbn_spmv``."""
+    """A complex sparse matrix in row blocks for ``tkbn_spmv``.
+
+    Rows are grouped in blocks of ``_BLOCK_ROWS``; a block keeps the union of its rows'
+    columns (``ucol``) and, per union column, the block's ``_BLOCK_ROWS`` values
+    (``uval``, zero where a row has no entry).  The union columns of each block are cut
+    into segments of at most ``_SEG_COLS`` (one wave each); blocks with several
+    segments add partial sums in a fix-up pass (``fix``).
+    """
 
     def __init__(self, rows: Tensor, cols: Tensor, vals: Tensor, nrows: int, ncols: int):
         dev = vals.device
+        R = _BLOCK_ROWS
         self.nrows, self.ncols = int(nrows), int(ncols)
+        if ncols >= 2**31:
+            raise ValueError("sparse interpolation supports at most 2^31 - 1 grid points")
         if rows.numel():
             lo = torch.stack([rows.min(), cols.min()]).tolist()
             hi = torch.stack([rows.max(), cols.max()]).tolist()
             if lo[0] < 0 or lo[1] < 0 or hi[0] >= nrows or hi[1] >= ncols:
                 raise ValueError(f"sparse matrix indices outside its shape ({nrows}, {ncols})")
-        # sort by (row, col), sum duplicates
+        # sum duplicate (row, col) entries
         key, perm = torch.sort(rows * ncols + cols, stable=True)
         vals = vals[perm]
         uniq, inverse = torch.unique_consecutive(key, return_inverse=True)
@@ -136,29 +149,36 @@ class SpmvPlan:
             summed = torch.zeros((uniq.numel(), 2), dtype=vals.real.dtype, device=dev)
             vals = torch.view_as_complex(summed.index_add_(0, inverse, torch.view_as_real(vals)))
         r = torch.div(uniq, ncols, rounding_mode="floor")
-        if ncols >= 2**31:
-            raise ValueError("sparse interpolation supports at most 2^31 - 1 grid points")
-        self.col = (uniq - r * ncols).to(torch.int32).contiguous()
-        self.val = vals.contiguous()
-        crow = torch.zeros(self.nrows + 1, dtype=torch.long, device=dev)
-        crow[1:] = torch.cumsum(torch.bincount(r, minlength=self.nrows), 0)
-        # row segments of at most _SEG_NNZ nonzeros (an empty row keeps one, empty)
-        n = crow[1:] - crow[:-1]
-        nseg_r = torch.clamp(torch.div(n + _SEG_NNZ - 1, _SEG_NNZ, rounding_mode="floor"), min=1)
-        ends = torch.cumsum(nseg_r, 0)
-        self.nseg = int(ends[-1].item()) if self.nrows else 0
-        first = ends - nseg_r
-        seg_row = torch.repeat_interleave(torch.arange(self.nrows, device=dev), nseg_r, output_size=self.nseg)
-        within = torch.arange(self.nseg, device=dev) - first[seg_row]
-        e0 = crow[seg_row] + within * _SEG_NNZ
-        e1 = torch.minimum(e0 + _SEG_NNZ, crow[seg_row + 1])
-        split = (nseg_r > 1)[seg_row]
-        slot = torch.where(split, torch.cumsum(split.long(), 0) - 1, torch.full_like(seg_row, -1))
-        self.seg = torch.stack([seg_row, e0, e1, slot], 1).contiguous()
+        c = uniq - r * ncols
+        # union columns per row block, with the block's values per union column
+        nblk = -(-self.nrows // R)
+        bkey, bperm = torch.sort(torch.div(r, R, rounding_mode="floor") * ncols + c, stable=True)
+        ukey, uinv = torch.unique_consecutive(bkey, return_inverse=True)
+        ublk = torch.div(ukey, ncols, rounding_mode="floor")
+        self.ucol = (ukey - ublk * ncols).to(torch.int32).contiguous()
+        uval = torch.zeros((ukey.numel(), R), dtype=vals.dtype, device=dev)
+        uval[uinv, r[bperm] % R] = vals[bperm]
+        self.uval = uval.contiguous()
+        ustart = torch.zeros(nblk + 1, dtype=torch.long, device=dev)
+        ustart[1:] = torch.cumsum(torch.bincount(ublk, minlength=nblk), 0)
+        # segments of at most _SEG_COLS union columns (an empty block keeps one, empty)
+        n = ustart[1:] - ustart[:-1]
+        nseg_b = torch.clamp(torch.div(n + _SEG_COLS - 1, _SEG_COLS, rounding_mode="floor"), min=1)
+        ends = torch.cumsum(nseg_b, 0)
+        self.nseg = int(ends[-1].item()) if nblk else 0
+        first = ends - nseg_b
+        seg_blk = torch.repeat_interleave(torch.arange(nblk, device=dev), nseg_b, output_size=self.nseg)
+        within = torch.arange(self.nseg, device=dev) - first[seg_blk]
+        u0 = ustart[seg_blk] + within * _SEG_COLS
+        u1 = torch.minimum(u0 + _SEG_COLS, ustart[seg_blk + 1])
+        split = (nseg_b > 1)[seg_blk]
+        slot = torch.where(split, torch.cumsum(split.long(), 0) - 1, torch.full_like(seg_blk, -1))
+        self.seg = torch.stack([seg_blk, u0, u1, slot], 1).contiguous()
         self.nslots = int(split.sum().item()) if self.nseg else 0
-        fix_rows = torch.nonzero(nseg_r > 1).reshape(-1)
-        self.fix = torch.stack([fix_rows, slot[first[fix_rows]], nseg_r[fix_rows]], 1).contiguous()
-        self.nfix = int(fix_rows.numel())
+        fix_blk = torch.nonzero(nseg_b > 1).reshape(-1)
+        self.fix = torch.stack([fix_blk, slot[first[fix_blk]], nseg_b[fix_blk]], 1).contiguous()
+        self.nfix = int(fix_blk.numel())
+        self.nnz = int(uniq.numel())
 
     def apply(self, x: Tensor) -> Tensor:
         """y (batch, nrows) = A x for x (batch, ncols), complex, on the plan's device."""
@@ -168,6 +188,8 @@ class SpmvPlan:
         if batch == 0 or self.nrows == 0:
             return y
         lib = _lib.load()
+        if lib.tkbn_spmv_block_rows() != _BLOCK_ROWS:
+            raise RuntimeError("libtkbn_hip was built with another sparse row-block size; rebuild")
         code = _lib.dtype_code(x)
         nbytes = lib.tkbn_spmv_workspace_bytes(code, self.nrows, self.ncols, batch, self.nslots)
         if nbytes < 0:
@@ -175,8 +197,8 @@ class SpmvPlan:
         ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=x.device)
         st = lib.tkbn_spmv(
             code, self.nrows, self.ncols, batch, self.nseg, self.seg.data_ptr(), self.nfix,
-            self.fix.data_ptr() if self.nfix else None, self.nslots, self.col.data_ptr(),
-            self.val.data_ptr(), x.data_ptr(), y.data_ptr(), ws.data_ptr(), nbytes, _lib.stream_of(x),
+            self.fix.data_ptr() if self.nfix else None, self.nslots, self.ucol.data_ptr(),
+            self.uval.data_ptr(), x.data_ptr(), y.data_ptr(), ws.data_ptr(), nbytes, _lib.stream_of(x),
         )
         _lib.check(st, "tkbn_spmv")
         return y

@@ -1,32 +1,40 @@
-// Sparse-matrix interpolation (gfx950): y[b] = A x[b] for a complex CSR matrix A and
-// a batch of complex vectors x[b] (the rows of x, one per (batch, coil) pair).
+// Sparse-matrix interpolation (gfx950): y[b] = A x[b] for a complex sparse matrix A
+// and a batch of complex vectors x[b] (the rows of x, one per (batch, coil) pair).
 //
 // Replaces the torch.mm calls of torchkbnufft/_nufft/interp.py:13-85
 // (spmat_interp / spmat_interp_adjoint).  The reference multiplies two real sparse
 // matrices (real and imaginary parts, spmat.py:92-103) with the transposed real and
 // imaginary parts of the data, four products per direction; here the host merges the
-// two into one complex CSR matrix (and its conjugate transpose for the adjoint), so a
+// two into one complex matrix (and its conjugate transpose for the adjoint), so a
 // direction is one pass over the matrix.
 //
-// Layout.  The vectors arrive batch-major, x (batch, ncols); a product gathers x at
-// scattered columns, so x is first transposed into the workspace as xt (ncols, batch):
-// every nonzero then reads one contiguous run of `batch` values.  Results are written
-// the same way into yt (nrows, batch) and transposed back into y (batch, nrows).  With
-// batch == 1 both transposes are skipped.
+// Matrix format (row blocks).  Rows are grouped in blocks of kSpRows consecutive rows;
+// a block stores the union of its rows' columns once (ucol) and, per union column,
+// the kSpRows values of that column (zero where a row has none).  Neighbouring rows of
+// an interpolation matrix share most columns (consecutive samples of a trajectory,
+// neighbouring grid points), so each x value a block needs is read once for all its
+// rows: the union of 8 radial samples' 6x6 footprints is ~80 columns instead of 288.
 //
-// Work split.  The host cuts every CSR row into segments of at most a fixed number of
-// nonzeros (a radial trajectory's centre grid points have thousands, the rest tens).
-// A wave takes one segment: its lanes cover (batch row, nonzero slot) pairs, partial
-// sums meet in a fixed butterfly, and the segment's sum goes straight into yt if the
-// row has one segment, or into a partial slot otherwise; a fix-up kernel adds the
-// partial slots of each split row in segment order.  No atomics: every output element
-// is summed in the same order on every call.
+// Layout.  The vectors arrive batch-major, x (batch, ncols); x is first transposed into
+// the workspace as xt (ncols, batch), so every union column reads one contiguous run
+// of `batch` values.  Results are written into yt (nrows, batch) and transposed back
+// into y (batch, nrows).  With batch == 1 both transposes are skipped.
+//
+// Work split.  The host cuts every block's union columns into segments of at most a
+// fixed length (a radial trajectory's centre grid points gather thousands of samples).
+// A wave takes one segment: its lanes cover (batch element, column slot) pairs, the
+// kSpRows sums of each lane meet the other slots' in a fixed butterfly, and the
+// segment's sums go straight into yt if the block has one segment, or into a partial
+// slot otherwise; a fix-up kernel adds the partial slots of each split block in
+// segment order.  No atomics: every output element is summed in the same order on
+// every call.
 #include "tkbn_common.h"
 
 namespace tkbn {
 
-constexpr int kSpWaves = 4;  // waves per workgroup
+constexpr int kSpWaves = 4;   // waves per workgroup
 constexpr int kSpChunk = 64;  // workgroups dealt to one XCD in a row
+constexpr int kSpRows = 8;    // rows per block (one value record per union column)
 
 // Workgroup b runs on XCD b mod 8; deal the logical blocks (consecutive rows) to the
 // XCDs in chunks of kSpChunk so that neighbouring rows, which read overlapping
@@ -54,82 +62,107 @@ __device__ __forceinline__ double bcast(double v, int j) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// seg: nseg x {row, first nonzero, end nonzero, partial slot or -1}.  A wave takes a
-// segment; lane (bl, k) covers batch element b0 + bl and nonzero slot k of K.  The
-// segment's columns and values are read 64 at a time by one coalesced load each and
-// handed to the lanes by cross-lane reads, so the x loads of a group of nonzeros
-// issue back to back.
+// seg: nseg x {block, first union column, end union column, partial slot or -1}.  A
+// wave takes a segment; lane (bl, k) covers batch element b0 + bl and column slot k of
+// K.  The segment's columns are read 64 at a time by one coalesced load and handed to
+// the lanes by cross-lane reads, so the x loads of a group of columns issue back to
+// back; each column's kSpRows values are one 64-byte record (uniform scalar loads
+// when K == 1).
 template <typename T, int LB>
 __global__ void __launch_bounds__(64 * kSpWaves)
-    spmv_seg_kernel(int64_t nseg, const int64_t* __restrict__ seg, const int* __restrict__ col,
-                    const cplx<T>* __restrict__ val, const cplx<T>* __restrict__ xt, int64_t batch,
+    spmv_seg_kernel(int64_t nseg, const int64_t* __restrict__ seg, const int* __restrict__ ucol,
+                    const cplx<T>* __restrict__ uval, const cplx<T>* __restrict__ xt, int64_t batch, int64_t nrows,
                     cplx<T>* __restrict__ yt, cplx<T>* __restrict__ part) {
-  constexpr int K = 64 / LB;  // nonzero slots per wave
-  constexpr int U = K == 1 ? 8 : 4;  // loads in flight per lane
+  constexpr int K = 64 / LB;          // column slots per wave
+  constexpr int U = K == 1 ? 4 : 2;   // x loads in flight per lane
   const int lane = threadIdx.x & 63;
   const int bl = lane % LB, k = lane / LB;
+  // wave-uniform in a scalar register: the segment, its columns and (K == 1) the value
+  // records then load through the scalar unit
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t nwaves = (int64_t)gridDim.x * kSpWaves;
-  for (int64_t s = spmv_block(blockIdx.x) * kSpWaves + (threadIdx.x >> 6); s < nseg; s += nwaves) {
-    const int64_t row = seg[4 * s], e0 = seg[4 * s + 1], e1 = seg[4 * s + 2], slot = seg[4 * s + 3];
-    cplx<T>* dst = slot < 0 ? yt + row * batch : part + slot * batch;
+  for (int64_t s = spmv_block(blockIdx.x) * kSpWaves + wave; s < nseg; s += nwaves) {
+    const int64_t blk = seg[4 * s], u0 = seg[4 * s + 1], u1 = seg[4 * s + 2], slot = seg[4 * s + 3];
     for (int64_t b0 = 0; b0 < batch; b0 += LB) {
       const int64_t b = b0 + bl;
       const cplx<T>* xb = xt + (b < batch ? b : batch - 1);  // idle lanes read an in-range element
-      T ar = 0, ai = 0;
-      for (int64_t c0 = e0; c0 < e1; c0 += 64) {
-        const int n = (int)(e1 - c0 < 64 ? e1 - c0 : 64);
-        const int64_t ei = c0 + (lane < n ? lane : 0);
-        const int ci = col[ei];
-        const T vr_i = val[ei].x, vi_i = val[ei].y;
-        // steps of U * K nonzeros; slots past n read column ci of lane >= n (a valid
-        // column) with weight 0
+      T ar[kSpRows], ai[kSpRows];
+#pragma unroll
+      for (int r = 0; r < kSpRows; ++r) ar[r] = ai[r] = T(0);
+      for (int64_t c0 = u0; c0 < u1; c0 += 64) {
+        const int n = (int)(u1 - c0 < 64 ? u1 - c0 : 64);
+        const int ci = ucol[c0 + (lane < n ? lane : 0)];
+        // steps of U * K columns; slots past n read the column of lane 63's load (a
+        // valid column) with weight 0
         for (int j = 0; j < n; j += U * K) {
           cplx<T> xv[U];
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             const int jj = j + u * K + k;
-            const int c = bcast<K>(ci, jj < 64 ? jj : 63);
-            xv[u] = xb[(int64_t)c * batch];
+            xv[u] = xb[(int64_t)bcast<K>(ci, jj < 64 ? jj : 63) * batch];
           }
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             const int jj = j + u * K + k;
-            const int jc = jj < 64 ? jj : 63;
-            // every lane takes part in the cross-lane reads (a lane outside a branch
-            // would hand out no value), then slots past n drop theirs
-            const T vr0 = bcast<K>(vr_i, jc), vi0 = bcast<K>(vi_i, jc);
             const bool on = jj < n;
-            const T vr = on ? vr0 : T(0), vi = on ? vi0 : T(0);
-            ar += vr * xv[u].x - vi * xv[u].y;
-            ai += vr * xv[u].y + vi * xv[u].x;
+            // one column per wave when K == 1: a wave-uniform index (scalar loads)
+            const int e = K == 1 ? __builtin_amdgcn_readfirstlane(on ? jj : 0) : (on ? jj : 0);
+            const cplx<T>* v = uval + (c0 + e) * kSpRows;
+#pragma unroll
+            for (int r = 0; r < kSpRows; ++r) {
+              const cplx<T> w = v[r];
+              const T wr = on ? w.x : T(0), wi = on ? w.y : T(0);
+              ar[r] += wr * xv[u].x - wi * xv[u].y;
+              ai[r] += wr * xv[u].y + wi * xv[u].x;
+            }
           }
         }
       }
 #pragma unroll
-      for (int off = LB; off < 64; off <<= 1) {
-        ar += __shfl_xor(ar, off);
-        ai += __shfl_xor(ai, off);
+      for (int r = 0; r < kSpRows; ++r) {
+#pragma unroll
+        for (int off = LB; off < 64; off <<= 1) {
+          ar[r] += __shfl_xor(ar[r], off);
+          ai[r] += __shfl_xor(ai[r], off);
+        }
       }
-      if (k == 0 && b < batch) dst[b] = make_c<T>(ar, ai);
+      if (k == 0 && b < batch) {
+#pragma unroll
+        for (int r = 0; r < kSpRows; ++r) {
+          const int64_t row = blk * kSpRows + r;
+          if (slot >= 0)
+            part[(slot * kSpRows + r) * batch + b] = make_c<T>(ar[r], ai[r]);
+          else if (row < nrows)
+            yt[row * batch + b] = make_c<T>(ar[r], ai[r]);
+        }
+      }
     }
   }
 }
 
-// fix: nfix x {row, first partial slot, slot count}; one wave per split row
+// fix: nfix x {block, first partial slot, slot count}; one wave per (split block, row
+// of the block), lanes over the batch, partial slots added in segment order
 template <typename T>
 __global__ void __launch_bounds__(64 * kSpWaves)
     spmv_fix_kernel(int64_t nfix, const int64_t* __restrict__ fix, const cplx<T>* __restrict__ part,
-                    int64_t batch, cplx<T>* __restrict__ yt) {
+                    int64_t batch, int64_t nrows, cplx<T>* __restrict__ yt) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * kSpWaves;
-  for (int64_t f = (int64_t)blockIdx.x * kSpWaves + (threadIdx.x >> 6); f < nfix; f += nwaves) {
-    const int64_t row = fix[3 * f], first = fix[3 * f + 1], cnt = fix[3 * f + 2];
+  for (int64_t it = (int64_t)blockIdx.x * kSpWaves + (threadIdx.x >> 6); it < nfix * kSpRows; it += nwaves) {
+    const int64_t f = it / kSpRows;
+    const int r = (int)(it - f * kSpRows);
+    const int64_t blk = fix[3 * f], first = fix[3 * f + 1], cnt = fix[3 * f + 2];
+    const int64_t row = blk * kSpRows + r;
+    if (row >= nrows) continue;
     for (int64_t b = lane; b < batch; b += 64) {
+      const cplx<T>* p = part + (first * kSpRows + r) * batch + b;
+      const int64_t step = (int64_t)kSpRows * batch;
       T ar = 0, ai = 0;
+#pragma unroll 8
       for (int64_t j = 0; j < cnt; ++j) {
-        const cplx<T> p = part[(first + j) * batch + b];
-        ar += p.x;
-        ai += p.y;
+        const cplx<T> v = p[j * step];
+        ar += v.x;
+        ai += v.y;
       }
       yt[row * batch + b] = make_c<T>(ar, ai);
     }
@@ -167,7 +200,7 @@ inline SpmvWs spmv_ws_layout(size_t esize, int64_t nrows, int64_t ncols, int64_t
   w.xt = 0;
   w.yt = w.xt + (tr ? align256(esize * ncols * batch) : 0);
   w.part = w.yt + (tr ? align256(esize * nrows * batch) : 0);
-  w.total = w.part + align256(esize * nslots * batch);
+  w.total = w.part + align256(esize * nslots * kSpRows * batch);
   return w;
 }
 
@@ -181,7 +214,7 @@ void launch_transpose(int64_t rows, int64_t cols, const void* src, void* dst, hi
 
 template <typename T>
 int run_spmv(int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const int64_t* seg, int64_t nfix,
-             const int64_t* fix, int64_t nslots, const int* col, const void* val, const void* x, void* y,
+             const int64_t* fix, int64_t nslots, const int* ucol, const void* uval, const void* x, void* y,
              char* ws, hipStream_t s) {
   const SpmvWs w = spmv_ws_layout(sizeof(cplx<T>), nrows, ncols, batch, nslots);
   const bool tr = batch > 1;
@@ -199,8 +232,9 @@ int run_spmv(int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const in
     int64_t nb = ceil_div(nseg, kSpWaves);
     nb = nb < (1 << 20) ? nb : (1 << 20);
     const unsigned blocks = (unsigned)(ceil_div(nb, 8 * kSpChunk) * 8 * kSpChunk);
-    const auto* v = static_cast<const cplx<T>*>(val);
-#define TKBN_SPMV(LBV) spmv_seg_kernel<T, LBV><<<blocks, dim3(64 * kSpWaves), 0, s>>>(nseg, seg, col, v, xt, batch, yt, part)
+    const auto* v = static_cast<const cplx<T>*>(uval);
+#define TKBN_SPMV(LBV) \
+  spmv_seg_kernel<T, LBV><<<blocks, dim3(64 * kSpWaves), 0, s>>>(nseg, seg, ucol, v, xt, batch, nrows, yt, part)
     if (batch == 1)
       TKBN_SPMV(1);
     else if (batch == 2)
@@ -219,8 +253,9 @@ int run_spmv(int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const in
     TKBN_LAUNCH_CHECK("spmv_seg_kernel");
   }
   if (nfix > 0) {
-    const unsigned blocks = (unsigned)(ceil_div(nfix, kSpWaves) < (1 << 20) ? ceil_div(nfix, kSpWaves) : (1 << 20));
-    spmv_fix_kernel<T><<<blocks, dim3(64 * kSpWaves), 0, s>>>(nfix, fix, part, batch, yt);
+    const int64_t items = ceil_div(nfix * kSpRows, kSpWaves);
+    const unsigned blocks = (unsigned)(items < (1 << 20) ? items : (1 << 20));
+    spmv_fix_kernel<T><<<blocks, dim3(64 * kSpWaves), 0, s>>>(nfix, fix, part, batch, nrows, yt);
     TKBN_LAUNCH_CHECK("spmv_fix_kernel");
   }
   if (tr) {
@@ -242,14 +277,16 @@ int64_t tkbn_spmv_workspace_bytes(int32_t dtype, int64_t nrows, int64_t ncols, i
   return (int64_t)spmv_ws_layout(es, nrows, ncols, batch, nslots).total;
 }
 
+int tkbn_spmv_block_rows(void) { return kSpRows; }
+
 int tkbn_spmv(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const int64_t* seg,
-              int64_t nfix, const int64_t* fix, int64_t nslots, const int32_t* col, const void* val, const void* x,
+              int64_t nfix, const int64_t* fix, int64_t nslots, const int32_t* ucol, const void* uval, const void* x,
               void* y, void* workspace, int64_t workspace_bytes, void* stream) {
   TKBN_REQUIRE(dtype == TKBN_F32 || dtype == TKBN_F64, TKBN_ERR_INVALID, "dtype must be TKBN_F32 or TKBN_F64");
   TKBN_REQUIRE(nrows >= 0 && ncols >= 0 && batch >= 0 && nseg >= 0 && nfix >= 0 && nslots >= 0, TKBN_ERR_INVALID,
                "negative size");
   if (nrows == 0 || batch == 0) return TKBN_OK;
-  TKBN_REQUIRE(nseg >= nrows, TKBN_ERR_INVALID, "every row needs at least one segment");
+  TKBN_REQUIRE(nseg >= ceil_div(nrows, kSpRows), TKBN_ERR_INVALID, "every row block needs at least one segment");
   TKBN_REQUIRE(ncols <= INT32_MAX, TKBN_ERR_UNSUPPORTED, "more than 2^31 - 1 columns");
   TKBN_REQUIRE(seg && y && (nfix == 0 || fix) && (ncols == 0 || x), TKBN_ERR_INVALID, "NULL pointer");
   const int64_t need = tkbn_spmv_workspace_bytes(dtype, nrows, ncols, batch, nslots);
@@ -258,8 +295,8 @@ int tkbn_spmv(int32_t dtype, int64_t nrows, int64_t ncols, int64_t batch, int64_
   hipStream_t s = static_cast<hipStream_t>(stream);
   char* ws = static_cast<char*>(workspace);
   if (dtype == TKBN_F32)
-    return run_spmv<float>(nrows, ncols, batch, nseg, seg, nfix, fix, nslots, col, val, x, y, ws, s);
-  return run_spmv<double>(nrows, ncols, batch, nseg, seg, nfix, fix, nslots, col, val, x, y, ws, s);
+    return run_spmv<float>(nrows, ncols, batch, nseg, seg, nfix, fix, nslots, ucol, uval, x, y, ws, s);
+  return run_spmv<double>(nrows, ncols, batch, nseg, seg, nfix, fix, nslots, ucol, uval, x, y, ws, s);
 }
 
 }  // extern "C"
