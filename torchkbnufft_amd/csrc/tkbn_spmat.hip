@@ -17,14 +17,15 @@
 //
 // Layout.  The vectors arrive batch-major, x (batch, ncols); x is first transposed into
 // the workspace as xt (ncols, batch), so every union column reads one contiguous run
-// of `batch` values.  Results are written into yt (nrows, batch) and transposed back
-// into y (batch, nrows).  With batch == 1 both transposes are skipped.
+// of `batch` values.  Results go straight into y (batch, nrows): a lane writes its
+// batch element's kSpRows consecutive rows as one 64-byte run.  With batch == 1 the
+// transpose is skipped.
 //
 // Work split.  The host cuts every block's union columns into segments of at most a
 // fixed length (a radial trajectory's centre grid points gather thousands of samples).
 // A wave takes one segment: its lanes cover (batch element, column slot) pairs, the
 // kSpRows sums of each lane meet the other slots' in a fixed butterfly, and the
-// segment's sums go straight into yt if the block has one segment, or into a partial
+// segment's sums go straight into y if the block has one segment, or into a partial
 // slot otherwise; a fix-up kernel adds the partial slots of each split block in
 // segment order.  No atomics: every output element is summed in the same order on
 // every call.
@@ -72,7 +73,7 @@ template <typename T, int LB>
 __global__ void __launch_bounds__(64 * kSpWaves)
     spmv_seg_kernel(int64_t nseg, const int64_t* __restrict__ seg, const int* __restrict__ ucol,
                     const cplx<T>* __restrict__ uval, const cplx<T>* __restrict__ xt, int64_t batch, int64_t nrows,
-                    cplx<T>* __restrict__ yt, cplx<T>* __restrict__ part) {
+                    cplx<T>* __restrict__ y, cplx<T>* __restrict__ part, int vec) {
   constexpr int K = 64 / LB;          // column slots per wave
   constexpr int U = K == 1 ? 4 : 2;   // x loads in flight per lane
   const int lane = threadIdx.x & 63;
@@ -127,13 +128,29 @@ __global__ void __launch_bounds__(64 * kSpWaves)
         }
       }
       if (k == 0 && b < batch) {
+        if (slot >= 0) {
 #pragma unroll
-        for (int r = 0; r < kSpRows; ++r) {
-          const int64_t row = blk * kSpRows + r;
-          if (slot >= 0)
-            part[(slot * kSpRows + r) * batch + b] = make_c<T>(ar[r], ai[r]);
-          else if (row < nrows)
-            yt[row * batch + b] = make_c<T>(ar[r], ai[r]);
+          for (int r = 0; r < kSpRows; ++r) part[(slot * kSpRows + r) * batch + b] = make_c<T>(ar[r], ai[r]);
+        } else {
+          cplx<T>* yb = y + b * nrows + blk * kSpRows;
+          if (vec && (blk + 1) * kSpRows <= nrows) {
+            // the block's rows of this batch element: 16-byte aligned (vec), vector stores
+            float4* y4 = reinterpret_cast<float4*>(yb);
+#pragma unroll
+            for (int q = 0; q < kSpRows * (int)sizeof(cplx<T>) / 16; ++q) {
+              constexpr int per = 16 / (int)sizeof(T) / 2;  // complex values per 16 bytes
+              T v[4];
+#pragma unroll
+              for (int h = 0; h < 2 * per; ++h) v[h] = (h & 1) ? ai[q * per + h / 2] : ar[q * per + h / 2];
+              float4 o;
+              memcpy(&o, v, 16);
+              y4[q] = o;
+            }
+          } else {
+#pragma unroll
+            for (int r = 0; r < kSpRows; ++r)
+              if (blk * kSpRows + r < nrows) yb[r] = make_c<T>(ar[r], ai[r]);
+          }
         }
       }
     }
@@ -145,7 +162,7 @@ __global__ void __launch_bounds__(64 * kSpWaves)
 template <typename T>
 __global__ void __launch_bounds__(64 * kSpWaves)
     spmv_fix_kernel(int64_t nfix, const int64_t* __restrict__ fix, const cplx<T>* __restrict__ part,
-                    int64_t batch, int64_t nrows, cplx<T>* __restrict__ yt) {
+                    int64_t batch, int64_t nrows, cplx<T>* __restrict__ y) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * kSpWaves;
   for (int64_t it = (int64_t)blockIdx.x * kSpWaves + (threadIdx.x >> 6); it < nfix * kSpRows; it += nwaves) {
@@ -164,7 +181,7 @@ __global__ void __launch_bounds__(64 * kSpWaves)
         ar += v.x;
         ai += v.y;
       }
-      yt[row * batch + b] = make_c<T>(ar, ai);
+      y[b * nrows + row] = make_c<T>(ar, ai);
     }
   }
 }
@@ -191,15 +208,14 @@ __global__ void __launch_bounds__(256)
 inline size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
 struct SpmvWs {
-  size_t xt, yt, part, total;
+  size_t xt, part, total;
 };
 
 inline SpmvWs spmv_ws_layout(size_t esize, int64_t nrows, int64_t ncols, int64_t batch, int64_t nslots) {
   SpmvWs w{};
   const bool tr = batch > 1;
   w.xt = 0;
-  w.yt = w.xt + (tr ? align256(esize * ncols * batch) : 0);
-  w.part = w.yt + (tr ? align256(esize * nrows * batch) : 0);
+  w.part = w.xt + (tr ? align256(esize * ncols * batch) : 0);
   w.total = w.part + align256(esize * nslots * kSpRows * batch);
   return w;
 }
@@ -219,12 +235,11 @@ int run_spmv(int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const in
   const SpmvWs w = spmv_ws_layout(sizeof(cplx<T>), nrows, ncols, batch, nslots);
   const bool tr = batch > 1;
   const cplx<T>* xt = static_cast<const cplx<T>*>(x);
-  cplx<T>* yt = static_cast<cplx<T>*>(y);
+  cplx<T>* yo = static_cast<cplx<T>*>(y);
   if (tr) {
     launch_transpose<T>(batch, ncols, x, ws + w.xt, s);  // (batch, ncols) -> (ncols, batch)
     TKBN_LAUNCH_CHECK("transpose_kernel");
     xt = reinterpret_cast<const cplx<T>*>(ws + w.xt);
-    yt = reinterpret_cast<cplx<T>*>(ws + w.yt);
   }
   cplx<T>* part = reinterpret_cast<cplx<T>*>(ws + w.part);
   if (nseg > 0) {
@@ -233,8 +248,10 @@ int run_spmv(int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const in
     nb = nb < (1 << 20) ? nb : (1 << 20);
     const unsigned blocks = (unsigned)(ceil_div(nb, 8 * kSpChunk) * 8 * kSpChunk);
     const auto* v = static_cast<const cplx<T>*>(uval);
+    // every batch row of y starts 16-byte aligned
+    const int vec = (reinterpret_cast<uintptr_t>(y) % 16 == 0) && ((nrows * (int64_t)sizeof(cplx<T>)) % 16 == 0);
 #define TKBN_SPMV(LBV) \
-  spmv_seg_kernel<T, LBV><<<blocks, dim3(64 * kSpWaves), 0, s>>>(nseg, seg, ucol, v, xt, batch, nrows, yt, part)
+  spmv_seg_kernel<T, LBV><<<blocks, dim3(64 * kSpWaves), 0, s>>>(nseg, seg, ucol, v, xt, batch, nrows, yo, part, vec)
     if (batch == 1)
       TKBN_SPMV(1);
     else if (batch == 2)
@@ -255,13 +272,10 @@ int run_spmv(int64_t nrows, int64_t ncols, int64_t batch, int64_t nseg, const in
   if (nfix > 0) {
     const int64_t items = ceil_div(nfix * kSpRows, kSpWaves);
     const unsigned blocks = (unsigned)(items < (1 << 20) ? items : (1 << 20));
-    spmv_fix_kernel<T><<<blocks, dim3(64 * kSpWaves), 0, s>>>(nfix, fix, part, batch, nrows, yt);
+    spmv_fix_kernel<T><<<blocks, dim3(64 * kSpWaves), 0, s>>>(nfix, fix, part, batch, nrows, yo);
     TKBN_LAUNCH_CHECK("spmv_fix_kernel");
   }
-  if (tr) {
-    launch_transpose<T>(nrows, batch, yt, y, s);  // (nrows, batch) -> (batch, nrows)
-    TKBN_LAUNCH_CHECK("transpose_kernel");
-  }
+
   return TKBN_OK;
 }
 
