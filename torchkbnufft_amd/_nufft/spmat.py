@@ -238,7 +238,7 @@ def _plan(interp_mats, adjoint: bool, like: Tensor) -> SpmvPlan:
     if ri.shape == ii.shape and torch.equal(ri, ii):  # calc_tensor_spmatrix's pairs share indices
         rows, cols = ri[0], ri[1]
         vals = torch.complex(rv.to(dev, rt), iv.to(dev, rt))
-    else:  # A = A_re + i A_im, entries summed when the CSR form is built
+    else:  # A = A_re + i A_im, entries summed when the block form is built
         rows, cols = torch.cat([ri[0], ii[0]]), torch.cat([ri[1], ii[1]])
         z_r = torch.zeros_like(rv, dtype=rt, device=dev)
         z_i = torch.zeros_like(iv, dtype=rt, device=dev)
@@ -255,7 +255,7 @@ def _plan(interp_mats, adjoint: bool, like: Tensor) -> SpmvPlan:
 
 
 def clear_spmat_cache() -> None:
-    """Drop every cached CSR form of interpolation matrices."""
+    """Drop every cached block form of interpolation matrices."""
     _PLANS.clear()
 
 
