@@ -57,9 +57,13 @@ __device__ __forceinline__ unsigned int cache_src_word(const CacheSrc& s, int64_
 }
 
 // Compare the words with the copy and store them over it (see the header comment).
+// `zero` (n ints, optional): a plan's count array, cleared on every call so that a
+// rebuild starts from zero counts without a fill launch (plans that use it read no
+// counts after their scan).
 __global__ void __launch_bounds__(256) cache_check_kernel(CacheHdr* h, const CacheSrc src,
                                                           unsigned int* __restrict__ copy, unsigned long long key,
-                                                          unsigned long long call) {
+                                                          unsigned long long call, int* __restrict__ zero,
+                                                          int64_t nzero) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     h->key_ok = (h->valid == 1 && h->key == key) ? 1 : 0;
     h->key = key;
@@ -75,6 +79,8 @@ __global__ void __launch_bounds__(256) cache_check_kernel(CacheHdr* h, const Cac
     copy[i] = w;
   }
   if (!__all(same) && (threadIdx.x & 63) == 0) h->stale = call;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += (int64_t)gridDim.x * blockDim.x)
+    zero[i] = 0;
 }
 
 // marks the header stale (a plan build that could not be enqueued completely)
@@ -146,8 +152,9 @@ inline int64_t plan_cache_bytes(const tkbn_interp_desc* d) {
 inline bool plan_reused(const tkbn_interp_desc* d) { return (d->flags & TKBN_PLAN_REUSE) != 0; }
 
 // Enqueues the check kernel; returns the header (device) for the plan kernels.
+// `zero` / `nzero`: an int array the check kernel clears (see cache_check_kernel).
 inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const int* tile, const void* omega, void* ws,
-                            hipStream_t s, CacheHdr** hdr) {
+                            hipStream_t s, CacheHdr** hdr, int* zero = nullptr, int64_t nzero = 0) {
   CacheHdr* h = static_cast<CacheHdr*>(ws);
   const CacheSrc src = plan_cache_src(d, omega);
   unsigned int* copy = reinterpret_cast<unsigned int*>(static_cast<char*>(ws) + kCacheHdrBytes);
@@ -155,7 +162,7 @@ inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const int* tile
   const unsigned long long call = ++calls;  // from 1: a zeroed header never holds it
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(src.total, 256), 1024));
   cache_check_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(h, src, copy, plan_cache_key(d, kind, tile),
-                                                                    call);
+                                                                    call, zero, nzero);
   TKBN_LAUNCH_CHECK("cache_check_kernel");
   *hdr = h;
   return TKBN_OK;

@@ -1123,7 +1123,7 @@ __global__ void __launch_bounds__(kGatThreads)
   const int* wstart = plan + tg.o_wstart;
   const int t = plan[tg.o_work + wi];
   const int split = (int)(wi - wstart[t]);
-  const int V = plan[tg.o_count + t];
+  const int V = plan[tg.o_start + t + 1] - plan[tg.o_start + t];  // (counts are cleared each call)
   const int i0 = split * tg.target;
   const int i1 = min(i0 + tg.target, V);
   if (i0 >= i1) return;  // no sample is based in this tile
@@ -1569,11 +1569,10 @@ static void profile_after(hipStream_t s) {
   g_profile = ProfileHook{};
 }
 
-// count -> scan -> scatter of the multi-coil spread plan (each exits at once on a cache hit)
+// count -> scan -> scatter of the multi-coil spread plan (each exits at once on a cache
+// hit); the check kernel has cleared the counts
 static int coil_spread_plan(const Geom<float>& g, const TileGeom& tg, const void* omega, int* wsi,
                             const CacheHdr* cache, hipStream_t s) {
-  int st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
-  if (st != TKBN_OK) return st;
   dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
   if (g.M > 0) {
     coil_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
@@ -1596,7 +1595,7 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
   int st = TKBN_OK;
   if (!plan_reused(d)) {
     CacheHdr* cache = nullptr;
-    st = plan_cache_begin(d, kPlanSpreadCoils, tg.T, omega, ws, s, &cache);
+    st = plan_cache_begin(d, kPlanSpreadCoils, tg.T, omega, ws, s, &cache, wsi + tg.o_count, tg.nkeys);
     if (st != TKBN_OK) return st;
     st = coil_spread_plan(g, tg, omega, wsi, cache, s);
     if (st != TKBN_OK) {
@@ -1761,11 +1760,10 @@ static int64_t gather_workspace_bytes(const TileGeom& tg) {
   return tg.o_rec_bytes + tg.maxent * (int64_t)sizeof(GatRec);
 }
 
-// count -> scan -> scatter of the multi-coil gather plan (each exits at once on a cache hit)
+// count -> scan -> scatter of the multi-coil gather plan (each exits at once on a cache
+// hit); the check kernel has cleared the counts
 static int gather_coils_plan(const Geom<float>& g, const TileGeom& tg, const void* omega, int* wsi,
                              const CacheHdr* cache, hipStream_t s) {
-  int st = launch_zero_fill_unless<int>(wsi, tg.plan_ints, cache, s);
-  if (st != TKBN_OK) return st;
   dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
   gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
   TKBN_LAUNCH_CHECK("gat_bin_count_kernel");
@@ -1784,7 +1782,7 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
   int st = TKBN_OK;
   if (!plan_reused(d)) {
     CacheHdr* cache = nullptr;
-    st = plan_cache_begin(d, kPlanGatherCoils, tg.T, omega, ws, s, &cache);
+    st = plan_cache_begin(d, kPlanGatherCoils, tg.T, omega, ws, s, &cache, wsi + tg.o_count, tg.NB);
     if (st != TKBN_OK) return st;
     st = gather_coils_plan(g, tg, omega, wsi, cache, s);
     if (st != TKBN_OK) {
