@@ -526,7 +526,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-iters", type=int, default=20)
-    ap.add_argument("--form-iters", type=int, default=5, help="iterations of the batched / gradient forms")
+    ap.add_argument("--form-iters", type=int, default=10, help="iterations of the batched / gradient forms")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
     ap.add_argument("--workload", default="sense2d", choices=["sense2d", "kooshball3d", "copy512", "spmat"],
                     help="sense2d: the headline SENSE pair (default); kooshball3d: the 3-D deterministic case; "
