@@ -56,14 +56,22 @@ __device__ __forceinline__ unsigned int cache_src_word(const CacheSrc& s, int64_
   return 0;
 }
 
-// Compare the words with the copy and store them over it (see the header comment).
-// `zero` (n ints, optional): a plan's count array, cleared on every call so that a
-// rebuild starts from zero counts without a fill launch (plans that use it read no
-// counts after their scan).
+// Int ranges the check kernel clears on every call: `copies` arrays of `n` ints,
+// `stride` ints apart (the count arrays of one or more plans; a rebuild then starts
+// from zero counts without a fill launch, and the plans read no counts after their
+// scan), and `n2` ints at `p2` (a per-call scratch word).
+struct CacheZero {
+  int* p = nullptr;
+  int64_t n = 0, stride = 0, copies = 1;
+  int* p2 = nullptr;
+  int64_t n2 = 0;
+};
+
+// Compare the words with the copy and store them over it (see the header comment),
+// and clear the CacheZero ranges.
 __global__ void __launch_bounds__(256) cache_check_kernel(CacheHdr* h, const CacheSrc src,
                                                           unsigned int* __restrict__ copy, unsigned long long key,
-                                                          unsigned long long call, int* __restrict__ zero,
-                                                          int64_t nzero) {
+                                                          unsigned long long call, const CacheZero z) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     h->key_ok = (h->valid == 1 && h->key == key) ? 1 : 0;
     h->key = key;
@@ -79,8 +87,13 @@ __global__ void __launch_bounds__(256) cache_check_kernel(CacheHdr* h, const Cac
     copy[i] = w;
   }
   if (!__all(same) && (threadIdx.x & 63) == 0) h->stale = call;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += (int64_t)gridDim.x * blockDim.x)
-    zero[i] = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < z.n * z.copies;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = i / z.n;
+    z.p[c * z.stride + (i - c * z.n)] = 0;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < z.n2; i += (int64_t)gridDim.x * blockDim.x)
+    z.p2[i] = 0;
 }
 
 // marks the header stale (a plan build that could not be enqueued completely)
@@ -89,13 +102,6 @@ __global__ void cache_invalidate_kernel(CacheHdr* h) {
     h->valid = 0;
     h->key_ok = 0;
   }
-}
-
-template <typename E>
-__global__ void __launch_bounds__(256) zero_fill_unless_kernel(E* __restrict__ p, int64_t n, const CacheHdr* h) {
-  if (cache_hit(h)) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = E(0);
 }
 
 inline unsigned long long fnv1a(unsigned long long h, const void* data, size_t n) {
@@ -152,9 +158,9 @@ inline int64_t plan_cache_bytes(const tkbn_interp_desc* d) {
 inline bool plan_reused(const tkbn_interp_desc* d) { return (d->flags & TKBN_PLAN_REUSE) != 0; }
 
 // Enqueues the check kernel; returns the header (device) for the plan kernels.
-// `zero` / `nzero`: an int array the check kernel clears (see cache_check_kernel).
+// `zero`: int ranges the check kernel clears (see CacheZero).
 inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const int* tile, const void* omega, void* ws,
-                            hipStream_t s, CacheHdr** hdr, int* zero = nullptr, int64_t nzero = 0) {
+                            hipStream_t s, CacheHdr** hdr, const CacheZero& zero = CacheZero()) {
   CacheHdr* h = static_cast<CacheHdr*>(ws);
   const CacheSrc src = plan_cache_src(d, omega);
   unsigned int* copy = reinterpret_cast<unsigned int*>(static_cast<char*>(ws) + kCacheHdrBytes);
@@ -162,7 +168,7 @@ inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const int* tile
   const unsigned long long call = ++calls;  // from 1: a zeroed header never holds it
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(src.total, 256), 1024));
   cache_check_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(h, src, copy, plan_cache_key(d, kind, tile),
-                                                                    call, zero, nzero);
+                                                                    call, zero);
   TKBN_LAUNCH_CHECK("cache_check_kernel");
   *hdr = h;
   return TKBN_OK;
@@ -174,13 +180,5 @@ inline int plan_cache_invalidate(void* ws, hipStream_t s) {
   return TKBN_OK;
 }
 
-template <typename E>
-inline int launch_zero_fill_unless(E* p, int64_t n, const CacheHdr* h, hipStream_t s) {
-  if (n <= 0) return TKBN_OK;
-  const int64_t blocks = std::min<int64_t>(ceil_div(n, 256), 2048);
-  zero_fill_unless_kernel<E><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(p, n, h);
-  TKBN_LAUNCH_CHECK("zero_fill_unless_kernel");
-  return TKBN_OK;
-}
 
 }  // namespace tkbn

@@ -401,8 +401,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = ((t / tg.bstride[d]) % tg.nb[d]) * tg.T[d];
 
-  const int V = plan[tg.o_count + t];
   const int cstart = plan[tg.o_start + t];
+  const int V = plan[tg.o_start + t + 1] - cstart;  // (counts are cleared each call)
   const int i0 = split * tg.target;
   const int i1 = min(i0 + tg.target, V);
 
@@ -1211,7 +1211,7 @@ __global__ void __launch_bounds__(256)
   const int* wstart = plan + tg.o_wstart;
   const int t = plan[tg.o_work + wi];
   const int split = (int)(wi - wstart[t]);
-  const int V = plan[tg.o_count + t];
+  const int V = plan[tg.o_start + t + 1] - plan[tg.o_start + t];  // (counts are cleared each call)
   const int i0 = split * tg.target;
   const int i1 = min(i0 + tg.target, V);
   if (i0 >= i1) return;  // empty tile: nothing to gather
@@ -1471,9 +1471,12 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
   const auto* om = static_cast<const T*>(omega);
   auto* gr = static_cast<cplx<T>*>(grid);
   if (det || std::is_same<T, float>::value) {
-    // fixed point; the float kernel only runs if kdata or omega is not finite
-    int st = launch_zero_fill<unsigned long long>(static_cast<unsigned long long*>(maxabs_slot), 1, s);
-    if (st != TKBN_OK) return st;
+    // fixed point; the float kernel only runs if kdata or omega is not finite.  The
+    // slot starts at zero: cleared by the plan-cache check, or here without one
+    if (!cache) {
+      const int st = launch_zero_fill<unsigned long long>(static_cast<unsigned long long*>(maxabs_slot), 1, s);
+      if (st != TKBN_OK) return st;
+    }
     const int64_t n = g.B * g.C * g.M * 2;
     const int64_t nom = (g.batched ? g.B : 1) * D * g.M;
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(std::max(n, nom), 1024), 256));
@@ -1595,7 +1598,10 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
   int st = TKBN_OK;
   if (!plan_reused(d)) {
     CacheHdr* cache = nullptr;
-    st = plan_cache_begin(d, kPlanSpreadCoils, tg.T, omega, ws, s, &cache, wsi + tg.o_count, tg.nkeys);
+    CacheZero z;
+    z.p = wsi + tg.o_count;
+    z.n = tg.nkeys;
+    st = plan_cache_begin(d, kPlanSpreadCoils, tg.T, omega, ws, s, &cache, z);
     if (st != TKBN_OK) return st;
     st = coil_spread_plan(g, tg, omega, wsi, cache, s);
     if (st != TKBN_OK) {
@@ -1641,19 +1647,19 @@ int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const v
   const bool reuse = plan_reused(d);
   CacheHdr* cache = nullptr;
   int st = TKBN_OK;
-  if (!reuse) {
-    st = plan_cache_begin(d, kind, tg.T, omega, ws, s, &cache);
-    if (st != TKBN_OK) return st;
-  }
   int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
   void* slot = reinterpret_cast<char*>(wsi) + nplans * tg.plan_ints * 4;
   if (!reuse) {
-    // counts must start at zero: clear the plan area unless the cached plan is reused
-    st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
-    if (st != TKBN_OK) {
-      plan_cache_invalidate(ws, s);
-      return st;
-    }
+    // the check kernel clears every plan's counts and the max|kdata| slot
+    CacheZero z;
+    z.p = wsi + tg.o_count;
+    z.n = tg.NB;
+    z.stride = tg.plan_ints;
+    z.copies = nplans;
+    z.p2 = static_cast<int*>(slot);
+    z.n2 = 2;
+    st = plan_cache_begin(d, kind, tg.T, omega, ws, s, &cache, z);
+    if (st != TKBN_OK) return st;
   }
   st = spread_dispatch<T>(g, tg, det, kdata, omega, wsi, slot, grid, lds, s, cache, d->ndim, max_numpoints(d));
   if (st != TKBN_OK && !reuse) plan_cache_invalidate(ws, s);
@@ -1699,13 +1705,13 @@ int interp_fwd_tiled_t(const tkbn_interp_desc* d, const void* grid, const void* 
   int st = TKBN_OK;
   int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
   if (!reuse) {
-    st = plan_cache_begin(d, kPlanFwdTiles, tg.T, omega, ws, s, &cache);
+    CacheZero z;  // every plan's counts
+    z.p = wsi + tg.o_count;
+    z.n = tg.NB;
+    z.stride = tg.plan_ints;
+    z.copies = nplans;
+    st = plan_cache_begin(d, kPlanFwdTiles, tg.T, omega, ws, s, &cache, z);
     if (st != TKBN_OK) return st;
-    st = launch_zero_fill_unless<int>(wsi, nplans * tg.plan_ints, cache, s);
-    if (st != TKBN_OK) {
-      plan_cache_invalidate(ws, s);
-      return st;
-    }
   }
   st = fwd_tiled_dispatch<T>(g, tg, grid, omega, wsi, kdata, s, cache, d->ndim, max_numpoints(d));
   if (st != TKBN_OK && !reuse) plan_cache_invalidate(ws, s);
@@ -1782,7 +1788,10 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
   int st = TKBN_OK;
   if (!plan_reused(d)) {
     CacheHdr* cache = nullptr;
-    st = plan_cache_begin(d, kPlanGatherCoils, tg.T, omega, ws, s, &cache, wsi + tg.o_count, tg.NB);
+    CacheZero z;
+    z.p = wsi + tg.o_count;
+    z.n = tg.NB;
+    st = plan_cache_begin(d, kPlanGatherCoils, tg.T, omega, ws, s, &cache, z);
     if (st != TKBN_OK) return st;
     st = gather_coils_plan(g, tg, omega, wsi, cache, s);
     if (st != TKBN_OK) {
