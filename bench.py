@@ -535,8 +535,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's sharding "
-            "over GPUs is described in DESIGN.md and not run here",
+            f"bench.py measures one GPU only (asked for --gpus {args.gpus}); sharding the path "
+            "over several GPUs (coil rows per GPU, one coil-sum reduction) is not run here",
             file=sys.stderr,
         )
         return 2
