@@ -42,6 +42,7 @@
  *                           this gives d/d omega for both of them.
  *   tkbn_profile_next_kernel <- no reference counterpart: measurement hook (HIP events
  *                           around the next binned interpolation kernel).
+ *   tkbn_last_interp_kernel <- no reference counterpart: test hook (which kernel ran).
  */
 #ifndef TKBN_HIP_H
 #define TKBN_HIP_H
@@ -75,7 +76,9 @@ enum tkbn_dtype {
 
 enum tkbn_adj_mode {
   TKBN_ADJ_ATOMIC = 0,       /* sample-parallel spread, float atomics (order varies run to run) */
-  TKBN_ADJ_DETERMINISTIC = 1 /* cell-ordered gather over a sorted plan: bitwise repeatable */
+  TKBN_ADJ_DETERMINISTIC = 1 /* binned spread with 64-bit fixed-point tile accumulators and no
+                                split tiles: the result does not depend on arrival order, so it
+                                is bitwise repeatable */
 };
 
 /*
@@ -105,9 +108,14 @@ typedef struct tkbn_interp_desc {
 
 /*
  * desc->flags: TKBN_PLAN_REUSE asserts that the workspace already holds the plan that
- * an earlier call with this workspace built for bitwise the same omega, tables and
- * geometry; the call then skips building and checking it (no plan kernels at all,
- * for HIP-graph replays of fixed trajectories).  Passing it otherwise is undefined.
+ * an earlier call with this workspace built for bitwise the same omega and tables; the
+ * call then skips building and checking it (no plan kernels at all, for HIP-graph
+ * replays of fixed trajectories).  The library keeps a host-side record of the plan
+ * geometry (plan kind, tile shape, grid, numpoints, oversampling, n_shift, klength)
+ * each workspace last had built; a reuse call whose geometry differs, or whose
+ * workspace never had a plan built, returns TKBN_ERR_INVALID without enqueuing
+ * anything.  Changed omega or table VALUES under the flag are the caller's error: the
+ * plan is used as it is (indices stay in range, results are those of the old values).
  */
 #define TKBN_PLAN_REUSE 1
 
@@ -132,6 +140,14 @@ TKBN_API int tkbn_interp_fwd(const tkbn_interp_desc* desc, const void* grid, con
  * its stream.  One-shot; either event may be NULL.
  */
 TKBN_API int tkbn_profile_next_kernel(void* start_event, void* stop_event);
+
+/*
+ * Name of the main interpolation kernel that the last tkbn_interp_fwd / tkbn_interp_adj /
+ * tkbn_interp_domega call of this thread enqueued (e.g. "gather_coils_kernel",
+ * "spread_coil_regs_kernel", "spread_tiles_kernel<fixed,deterministic>"); "" before
+ * the first.  Lets tests assert which kernel a geometry takes.
+ */
+TKBN_API const char* tkbn_last_interp_kernel(void);
 
 /* Bytes of device workspace tkbn_interp_adj needs for `mode` (see the header note). */
 TKBN_API int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mode);

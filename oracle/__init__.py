@@ -29,6 +29,9 @@ Parity pins (what checks each function; see ``tests/test_oracle.py``)
 * ``table_path`` forward/adjoint/NUFFT: golden vectors from the reference in
   complex64 and float64 (``tests/golden/*.npz``); dot-product adjointness;
   FP64 NUDFT within the table-quantisation error; dense brute force on tiny sizes.
-* ``nudft``: numpy FFT on a uniform grid (closed form special case); analytic
-  derivative vs FP64 central differences.
+* ``nudft``: samples on the uniform grid reduce it to numpy's FFT times the shift
+  phase, and its adjoint to the unnormalised inverse FFT (``test_nudft_on_grid_is_shifted_fft``,
+  1e-12); ``nudft_domega`` against FP64 central differences of ``nudft``.
+* ``nudft.exact_kb_interp``: agreement with the table path to the table-quantisation
+  error, and its derivative against central differences.
 """

@@ -406,8 +406,8 @@ def spmat_interp_adjoint(data, interp_mats, grid_size):
 
 
 def inner_product(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
-    """``<a, b> = sum(conj(b) * a)`` (``_math.py:138-166``)."""
-    return torch.sum(b.conj() * a)
+    """``sum(b * conj(a))``: ``_math.py:138-166`` evaluates ``conj_complex_mult(val2, val1)``."""
+    return torch.sum(b * a.conj())
 
 
 def rel_l2(a, b) -> float:

@@ -178,13 +178,51 @@ def test_dcomp_and_toeplitz_validation_matches_reference_errors():
 def test_math_helpers():
     a = torch.randn(3, 4, dtype=torch.complex128)
     b = torch.randn(3, 4, dtype=torch.complex128)
-    assert torch.allclose(tkbn.inner_product(a, b), torch.sum(a * b.conj()))
+    # the reference returns sum(val2 * conj(val1)) (_math.py:160, conj_complex_mult(val2, val1))
+    assert torch.allclose(tkbn.inner_product(a, b), torch.sum(b * a.conj()))
+    e0 = torch.zeros(2, dtype=torch.complex128)
+    e0[0] = 1.0
+    assert complex(tkbn.inner_product(e0 * 1j, e0 * 2.0)) == pytest.approx(-2j)  # conj on the first argument
     ar, br = torch.view_as_real(a), torch.view_as_real(b)
     ip = tkbn.inner_product(ar, br)
-    assert torch.allclose(torch.complex(ip[0], ip[1]), torch.sum(a * b.conj()))
+    assert torch.allclose(torch.complex(ip[0], ip[1]), torch.sum(b * a.conj()))
     assert torch.allclose(torch.view_as_complex(tkbn.complex_mult(ar, br)), a * b)
     assert torch.allclose(torch.view_as_complex(tkbn.conj_complex_mult(ar, br)), a * b.conj())
     assert torch.allclose(tkbn.imag_exp(torch.tensor([0.5])), torch.exp(1j * torch.tensor([0.5])))
+    # imag_exp's real form stacks (cos, sin) last whatever dim is (_math.py:133)
+    assert tkbn.imag_exp(torch.zeros(3, 5), dim=0, return_complex=False).shape == (3, 5, 2)
+
+
+def test_complex_sign_matches_atan2_definition():
+    """complex_sign = exp(i atan2(im, re)) (_math.py:60-82): 1 at 0, -1 at -0.0, unit elsewhere."""
+    v = torch.tensor([0.0 + 0.0j, 3.0 + 4.0j, -2.0 + 0.0j, 0.0 - 5.0j], dtype=torch.complex128)
+    s = tkbn.complex_sign(v)
+    assert torch.allclose(s, torch.tensor([1.0, 0.6 + 0.8j, -1.0, -1j], dtype=torch.complex128))
+    neg0 = torch.view_as_complex(torch.tensor([[-0.0, 0.0]], dtype=torch.float64))
+    assert torch.allclose(tkbn.complex_sign(neg0), torch.tensor([-1.0 + 0j], dtype=torch.complex128))
+    r = torch.view_as_real(v)  # real input, (re, im) along the last dim
+    assert torch.allclose(torch.view_as_complex(tkbn.complex_sign(r).contiguous()), s)
+    with pytest.raises(ValueError):
+        tkbn.complex_sign(torch.zeros(4, 3))
+
+
+@pytest.mark.parametrize("rows,ndim,mode", [(1, 2, 0), (64, 2, 0), (64, 2, 1), (3, 3, 1), (1, 1, 0)])
+def test_plan_reuse_refused_for_unbuilt_workspace(rows, ndim, mode):
+    """TKBN_PLAN_REUSE on a workspace that holds no plan for this geometry is refused
+    with TKBN_ERR_INVALID before anything is enqueued (the pointer is never touched)."""
+    lib = _lib.load()
+    d = _desc(ndim=ndim, M=5000)
+    d.coils = rows
+    for i in range(3):
+        d.grid_size[i] = 640 if ndim == 2 else 64
+    d.flags = _lib.TKBN_PLAN_REUSE
+    fake_ws = 1 << 40  # 16-byte aligned, never dereferenced
+    big = 1 << 40
+    assert lib.tkbn_interp_adj(ctypes.byref(d), 1, 1, 1, mode, fake_ws, big, None) == _lib.TKBN_ERR_INVALID
+    assert b"TKBN_PLAN_REUSE" in lib.tkbn_last_error()
+    if lib.tkbn_interp_fwd_workspace_bytes(ctypes.byref(d)) > 0:
+        assert lib.tkbn_interp_fwd(ctypes.byref(d), 1, 1, 1, fake_ws, big, None) == _lib.TKBN_ERR_INVALID
+        assert b"TKBN_PLAN_REUSE" in lib.tkbn_last_error()
 
 
 @pytest.mark.parametrize("ndim", [1, 2, 3])

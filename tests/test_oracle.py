@@ -154,6 +154,29 @@ def test_oracle_dot_product(im_size, smaps):
     assert torch.allclose(lhs, rhs)
 
 
+@pytest.mark.parametrize("im_size", [(16,), (12, 10), (6, 8, 5)])
+def test_nudft_on_grid_is_shifted_fft(im_size):
+    """Samples on the uniform grid, omega = 2 pi k / N: the NUDFT is numpy's FFT times
+    the shift phase exp(i 2 pi k . s / N), and its adjoint is the unnormalised inverse
+    FFT times the conjugate phase (closed form, 1e-12)."""
+    rng = np.random.default_rng(5)
+    d = len(im_size)
+    n_shift = [n // 2 for n in im_size]
+    x = rng.standard_normal((2,) + im_size) + 1j * rng.standard_normal((2,) + im_size)
+    ks = np.meshgrid(*[np.arange(n) for n in im_size], indexing="ij")
+    k = np.stack([kk.reshape(-1) for kk in ks])  # (d, P) frequency indices, row-major
+    omega = 2 * np.pi * k / np.array(im_size, dtype=np.float64)[:, None]
+    phase = np.exp(1j * 2 * np.pi * np.sum(k * (np.array(n_shift)[:, None] / np.array(im_size)[:, None]), axis=0))
+    axes = tuple(range(1, d + 1))
+    ref = np.fft.fftn(x, axes=axes).reshape(2, -1) * phase
+    y = nudft.nudft(x, omega, n_shift)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-12
+    yy = rng.standard_normal((2, k.shape[1])) + 1j * rng.standard_normal((2, k.shape[1]))
+    adj = nudft.nudft_adjoint(yy, omega, im_size, n_shift)
+    ref_adj = np.fft.ifftn((yy * np.conj(phase)).reshape((2,) + im_size), axes=axes) * np.prod(im_size)
+    assert np.linalg.norm(adj - ref_adj) / np.linalg.norm(ref_adj) < 1e-12
+
+
 def test_nudft_domega_matches_fd():
     rng = np.random.default_rng(4)
     x = rng.standard_normal((1, 6, 7)) + 1j * rng.standard_normal((1, 6, 7))

@@ -66,6 +66,7 @@ _SIGNATURES = {
     ),
     "tkbn_interp_domega": (ctypes.c_int, [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "tkbn_profile_next_kernel": (ctypes.c_int, [_c_vp, _c_vp]),
+    "tkbn_last_interp_kernel": (ctypes.c_char_p, []),
     "tkbn_apod_pad": (
         ctypes.c_int,
         [_c_i32, _c_i32, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i64), _c_i64, _c_i64, _c_i64,

@@ -83,7 +83,11 @@ class BatchedKbNufft(torch.nn.Module):
         for g, members in enumerate(self.groups):
             for slot, p in enumerate(members):
                 self.where[p] = (g, slot)
-        self.ktrajs = [k.to(device=device, dtype=torch.float32).contiguous() for k in ktrajs]
+        # private copies: the plans are built once and trusted, so a caller's later
+        # in-place edit of its trajectory must not reach them
+        self.ktrajs = [
+            k.detach().to(device=device, dtype=torch.float32).clone(memory_format=torch.contiguous_format) for k in ktrajs
+        ]
         for k, s in zip(self.ktrajs, self.sizes):
             if k.ndim != 2 or k.shape[0] != len(s):
                 raise ValueError(f"trajectory of shape {tuple(k.shape)} for a {len(s)}-d image")

@@ -20,6 +20,9 @@
 namespace tkbn {
 
 void set_error(const char* fmt, ...);
+// Records the name of the interpolation kernel a call has just enqueued (thread-local;
+// read back with tkbn_last_interp_kernel).
+void note_kernel(const char* name);
 
 #define TKBN_REQUIRE(cond, code, ...)     \
   do {                                    \

@@ -21,6 +21,8 @@
 namespace tkbn {
 
 static thread_local char g_err[512];
+static thread_local const char* g_kernel = "";
+void note_kernel(const char* name) { g_kernel = name; }
 
 void set_error(const char* fmt, ...) {
   va_list ap;
@@ -225,6 +227,7 @@ int fwd_launch(const Geom<T>& g, const void* grid, const void* omega, void* kdat
   interp_fwd_kernel<T, D, JM><<<grd, dim3(256), 0, s>>>(
       g, static_cast<const cplx<T>*>(grid), static_cast<const T*>(omega), static_cast<cplx<T>*>(kdata), rpc);
   TKBN_LAUNCH_CHECK("interp_fwd_kernel");
+  note_kernel("interp_fwd_kernel");
   return TKBN_OK;
 }
 
@@ -236,6 +239,7 @@ int domega_launch(const Geom<T>& g, const void* grid, const void* gout, const vo
       g, static_cast<const cplx<T>*>(grid), static_cast<const cplx<T>*>(gout), static_cast<const T*>(omega),
       static_cast<T*>(out));
   TKBN_LAUNCH_CHECK("interp_domega_kernel");
+  note_kernel("interp_domega_kernel");
   return TKBN_OK;
 }
 
@@ -266,6 +270,8 @@ extern "C" {
 int tkbn_abi_version(void) { return TKBN_ABI_VERSION; }
 
 const char* tkbn_last_error(void) { return g_err; }
+
+const char* tkbn_last_interp_kernel(void) { return g_kernel; }
 
 int64_t tkbn_interp_fwd_workspace_bytes(const tkbn_interp_desc* desc) {
   if (validate_desc(desc) != TKBN_OK) return -1;

@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
+#include <unordered_map>
 
 #include "tkbn_common.h"
 
@@ -157,6 +159,39 @@ inline int64_t plan_cache_bytes(const tkbn_interp_desc* d) {
 // True when the caller asserted that the workspace's plan is current (no plan work).
 inline bool plan_reused(const tkbn_interp_desc* d) { return (d->flags & TKBN_PLAN_REUSE) != 0; }
 
+// Host-side record of the plan each workspace holds: the key of the last checked call
+// that enqueued its plan kernels through that workspace.  A TKBN_PLAN_REUSE call is
+// refused (before anything is enqueued) unless the record holds its own key, so a
+// reuse can never read a plan laid out for another geometry or tile shape.  One
+// process-wide map (inline function statics are shared by every translation unit).
+struct PlanRegistry {
+  std::mutex mu;
+  std::unordered_map<uintptr_t, unsigned long long> key_of;
+};
+inline PlanRegistry& plan_registry() {
+  static PlanRegistry r;
+  return r;
+}
+inline void plan_registry_note(const void* ws, unsigned long long key) {
+  PlanRegistry& r = plan_registry();
+  std::lock_guard<std::mutex> lock(r.mu);
+  if (key == 0)
+    r.key_of.erase(reinterpret_cast<uintptr_t>(ws));
+  else
+    r.key_of[reinterpret_cast<uintptr_t>(ws)] = key;
+}
+inline int plan_registry_require(const tkbn_interp_desc* d, int kind, const int* tile, const void* ws) {
+  const unsigned long long key = plan_cache_key(d, kind, tile);
+  PlanRegistry& r = plan_registry();
+  std::lock_guard<std::mutex> lock(r.mu);
+  const auto it = r.key_of.find(reinterpret_cast<uintptr_t>(ws));
+  TKBN_REQUIRE(it != r.key_of.end() && it->second == key, TKBN_ERR_INVALID,
+               "TKBN_PLAN_REUSE: workspace %p holds no plan built for this geometry (plan kind, tile shape, grid, "
+               "numpoints, oversampling, n_shift, klength); make one call without the flag first",
+               ws);
+  return TKBN_OK;
+}
+
 // Enqueues the check kernel; returns the header (device) for the plan kernels.
 // `zero`: int ranges the check kernel clears (see CacheZero).
 inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const int* tile, const void* omega, void* ws,
@@ -171,10 +206,12 @@ inline int plan_cache_begin(const tkbn_interp_desc* d, int kind, const int* tile
                                                                     call, zero);
   TKBN_LAUNCH_CHECK("cache_check_kernel");
   *hdr = h;
+  plan_registry_note(ws, plan_cache_key(d, kind, tile));
   return TKBN_OK;
 }
 
 inline int plan_cache_invalidate(void* ws, hipStream_t s) {
+  plan_registry_note(ws, 0);
   cache_invalidate_kernel<<<1, 64, 0, s>>>(static_cast<CacheHdr*>(ws));
   TKBN_LAUNCH_CHECK("cache_invalidate_kernel");
   return TKBN_OK;

@@ -1489,9 +1489,11 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
     TKBN_LAUNCH_CHECK("spread_tiles_kernel<fixed>");
     spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(256), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 1);
     TKBN_LAUNCH_CHECK("spread_tiles_kernel<non-finite>");
+    note_kernel(det ? "spread_tiles_kernel<fixed,deterministic>" : "spread_tiles_kernel<fixed>");
   } else {
     spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(256), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 0);
     TKBN_LAUNCH_CHECK("spread_tiles_kernel");
+    note_kernel("spread_tiles_kernel<float>");
   }
   return TKBN_OK;
 }
@@ -1610,6 +1612,9 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
       plan_cache_invalidate(ws, s);
       return st;
     }
+  } else {
+    st = plan_registry_require(d, kPlanSpreadCoils, tg.T, ws);
+    if (st != TKBN_OK) return st;
   }
   const int64_t nrows = g.B * g.C;
   const unsigned ychunks = (unsigned)ceil_div(nrows, 64);
@@ -1623,6 +1628,7 @@ static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, con
       g, tg, static_cast<const float2*>(kdata), wsi, static_cast<float2*>(grid), nchunk);
   profile_after(s);
   TKBN_LAUNCH_CHECK("spread_coil_regs_kernel");
+  note_kernel("spread_coil_regs_kernel");
   return TKBN_OK;
 }
 
@@ -1662,6 +1668,9 @@ int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const v
     z.n2 = 2;
     st = plan_cache_begin(d, kind, tg.T, omega, ws, s, &cache, z);
     if (st != TKBN_OK) return st;
+  } else {
+    st = plan_registry_require(d, kind, tg.T, ws);
+    if (st != TKBN_OK) return st;
   }
   st = spread_dispatch<T>(g, tg, det, kdata, omega, wsi, slot, grid, lds, s, cache, d->ndim, max_numpoints(d));
   if (st != TKBN_OK && !reuse) plan_cache_invalidate(ws, s);
@@ -1687,6 +1696,7 @@ int fwd_tiled_launch(const Geom<T>& g, const TileGeom& tg, const void* grid, con
       <<<dim3((unsigned)tg.maxwork, (unsigned)ceil_div(nrows, tg.CC), (unsigned)zdim), dim3(256), lds, s>>>(
           g, tg, static_cast<const cplx<T>*>(grid), static_cast<const T*>(omega), ws, static_cast<cplx<T>*>(kdata));
   TKBN_LAUNCH_CHECK("interp_fwd_tiled_kernel");
+  note_kernel("interp_fwd_tiled_kernel");
   return TKBN_OK;
 }
 
@@ -1713,6 +1723,9 @@ int interp_fwd_tiled_t(const tkbn_interp_desc* d, const void* grid, const void* 
     z.stride = tg.plan_ints;
     z.copies = nplans;
     st = plan_cache_begin(d, kPlanFwdTiles, tg.T, omega, ws, s, &cache, z);
+    if (st != TKBN_OK) return st;
+  } else {
+    st = plan_registry_require(d, kPlanFwdTiles, tg.T, ws);
     if (st != TKBN_OK) return st;
   }
   st = fwd_tiled_dispatch<T>(g, tg, grid, omega, wsi, kdata, s, cache, d->ndim, max_numpoints(d));
@@ -1800,6 +1813,9 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
       plan_cache_invalidate(ws, s);
       return st;
     }
+  } else {
+    st = plan_registry_require(d, kPlanGatherCoils, tg.T, ws);
+    if (st != TKBN_OK) return st;
   }
   const int nchunk = (int)ceil_div(g.B * g.C, kGatRows);
   const int64_t groups = ceil_div(tg.maxwork * nchunk, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
@@ -1813,6 +1829,7 @@ static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, cons
         g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata), nchunk);
   profile_after(s);
   TKBN_LAUNCH_CHECK("gather_coils_kernel");
+  note_kernel("gather_coils_kernel");
   return TKBN_OK;
 }
 
