@@ -134,6 +134,43 @@ def time_stages(ops, x, ktraj, smaps, iters: int):
     return stages, kernels
 
 
+def plan_build_us(fw, x, ktraj, smaps, reps: int = 5) -> dict:
+    """Trajectory-plan cost: a forward and an adjoint interpolation on a trajectory the
+    library has not seen (the bins, permutation and work lists are built) against the
+    same call again (plan reused after the on-device check), HIP events on the current
+    stream, median over `reps` new trajectories (the radial one scaled by 1 - r 1e-6)."""
+    from torchkbnufft_amd._nufft import fft as _fft
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    tabs = fw._tables()
+    g = _fft.fft_grid_fwd(x, smaps, fw.scaling_coef, (N, N), (K, K), 1.0)
+    args = (tabs, fw.n_shift, fw.numpoints, fw.table_oversamp, fw.offsets)
+    kd = _interp.table_interp(g, ktraj, *args)
+    res = {"fwd_first": [], "fwd_reused": [], "adj_first": [], "adj_reused": []}
+    for r in range(reps):
+        # values differ from every earlier call's, so the on-device check sees a new
+        # trajectory even where the allocator hands back an earlier copy's address
+        om = ktraj * (1.0 - 1e-6 * (r + 1))
+        for name, fn in (("fwd", lambda: _interp.table_interp(g, om, *args, out=kd)),
+                         ("adj", lambda: _interp.table_interp_adjoint(kd, om, *args, (K, K), out=g))):
+            for when in ("first", "reused"):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                fn()
+                e1.record()
+                torch.cuda.synchronize()
+                res[f"{name}_{when}"].append(e0.elapsed_time(e1) * 1e3)
+        del om
+    med = {k: float(np.median(v)) for k, v in res.items()}
+    return {
+        "fwd_call_first_us": med["fwd_first"],
+        "fwd_call_reused_us": med["fwd_reused"],
+        "adj_call_first_us": med["adj_first"],
+        "adj_call_reused_us": med["adj_reused"],
+        "plan_build_us": med["fwd_first"] - med["fwd_reused"] + med["adj_first"] - med["adj_reused"],
+    }
+
+
 def cpu_baseline(x, ktraj, smaps, seconds: float) -> dict:
     from oracle import table_path as tp
 
@@ -178,6 +215,16 @@ def vd_spiral(n_inter: int, n_read: int, turns: float = 16.0) -> torch.Tensor:
     return torch.tensor(np.concatenate(arms, axis=1), dtype=torch.float32)
 
 
+def nudft_rel_err(y_sel, image, om_sel, n_shift) -> float:
+    """Relative L2 error of GPU samples y_sel against the FP64 direct NUDFT of `image`
+    (one coil, already multiplied by its map) at the trajectory columns om_sel."""
+    from oracle import nudft
+
+    ref = nudft.nudft(image.detach().cpu().numpy().astype(np.complex128), om_sel.double().cpu().numpy(), n_shift)
+    y = y_sel.detach().cpu().numpy()
+    return float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+
+
 def timed(fn, iters: int) -> float:
     """Mean ms per call of fn (synchronised wall clock around `iters` calls)."""
     fn()
@@ -220,6 +267,14 @@ def bench_batched(dev, iters: int) -> dict:
     out = bt.replay()
     torch.cuda.synchronize()
     diff = max(float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in zip(out, ref))
+    # FP64 NUDFT error of the batched forward: the first problem of each size, coil 0,
+    # every 37th sample
+    fp64 = {}
+    for s in sorted(set(sizes)):
+        p = sizes.index(s)
+        sel = torch.arange(0, kt[p].shape[1], 37, device=dev)
+        fp64[f"{s[0]}x{s[1]}"] = nudft_rel_err(bt.kdata[p][0, 0, sel], images[p][0, 0], kt[p][:, sel],
+                                                (s[0] // 2, s[1] // 2))
     return {
         "workload": "64 problems, N in {128,192,256}^2 (uniform), 8 coils, VD spiral 8-48 interleaves x 1024 "
         "(r = pi t^1.5, 16 turns), fwd + adj of every problem per step",
@@ -229,6 +284,7 @@ def bench_batched(dev, iters: int) -> dict:
         "batched_graph_ms": t_graph,
         "speedup_graph_vs_loop": t_loop / t_graph,
         "max_rel_diff_vs_loop": diff,
+        "fp64_oracle_rel_err": fp64,
     }
 
 
@@ -269,6 +325,10 @@ def bench_grad(dev, iters: int) -> dict:
     gx, gk = grads(False)
     lx, lk = grads(True)
     torch.cuda.synchronize()
+    sel = torch.arange(0, ktraj.shape[1], 97, device=dev)
+    with torch.no_grad():
+        y0 = fw(x_true, ktraj, smaps=smaps[:, :1])[0, 0, sel]
+    fp64 = nudft_rel_err(y0, (x_true * smaps[:, :1])[0, 0], ktraj[:, sel], (n // 2, n // 2))
     return {
         "workload": "256^2, 12 coils (SENSE maps), 64-interleave VD spiral x 1536 (98304 samples); "
         "loss ||A(x, ktraj) - y||^2, y = A(x_true) + N(0, 0.01); d/dx and d/dktraj",
@@ -277,6 +337,7 @@ def bench_grad(dev, iters: int) -> dict:
         "speedup_vs_loop": t_loop / t_fused,
         "rel_diff_dx": float(torch.linalg.norm(gx - lx) / torch.linalg.norm(lx)),
         "rel_diff_dktraj": float(torch.linalg.norm(gk - lk) / torch.linalg.norm(lk)),
+        "fp64_oracle_rel_err": fp64,
     }
 
 
@@ -417,10 +478,12 @@ def bench_small(dev, iters: int, with_cpu: bool) -> dict:
     ad = tkbn.KbNufftAdjoint(im_size=(256, 256)).to(dev)
     xd, od = x.to(dev), om.to(dev)
     ms = timed(lambda: ad(fw(xd, od), od), iters)
+    sel = torch.arange(0, od.shape[1], 101, device=dev)
     out = {
         "workload": "1 coil, 256x256 -> 512x512, 402 x 512 radial (205824 samples), fwd + adj",
         "gpu_ms": ms,
         "gpu_pairs_per_s": 1e3 / ms,
+        "fp64_oracle_rel_err": nudft_rel_err(fw(xd, od)[0, 0, sel], xd[0, 0], od[:, sel], (128, 128)),
     }
     if with_cpu:
         from oracle import table_path as tp
@@ -576,12 +639,18 @@ def main() -> int:
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # one HIP event between consecutive steps on the stream the kernels run on: the
+    # per-step durations give the median / p10 / p90 (recording an event enqueues no work)
+    marks = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    marks[0].record()
+    for i in range(args.steps):
         step()
+        marks[i + 1].record()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms = elapsed / args.steps * 1e3
+    per_step = np.array([marks[i].elapsed_time(marks[i + 1]) for i in range(args.steps)])
 
     F = footprint_cells(ktraj_cpu.numpy(), (K, K), J)
     sb = stage_bytes(F, M)
@@ -600,6 +669,13 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
+        "step_ms_events": {
+            "median": float(np.median(per_step)),
+            "p10": float(np.percentile(per_step, 10)),
+            "p90": float(np.percentile(per_step, 90)),
+            "n": int(per_step.size),
+            "pairs_per_s_at_median": float(1e3 / np.median(per_step)),
+        },
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -634,6 +710,7 @@ def main() -> int:
         "kernels_us": {k: round(v, 2) for k, v in kernels.items()},
         "stage_bytes": sb,
         "fp64_oracle_rel_err": fp64_oracle_error(fw, x, ktraj, smaps),
+        "plan": plan_build_us(fw, x, ktraj, smaps),
     }
     if not args.no_forms:
         out["batched"] = bench_batched(dev, args.form_iters)
@@ -642,6 +719,14 @@ def main() -> int:
         out["toeplitz"] = bench_toeplitz(dev, x, ktraj, smaps, max(args.form_iters, 10), ms)
         out["small"] = bench_small(dev, max(args.form_iters, 20), not args.no_cpu_baseline)
         out["spmat"] = bench_spmat(dev, x, ktraj, smaps, args.form_iters, ms, not args.no_cpu_baseline)
+        out["kooshball"] = bench_kooshball(dev, 3, 1)
+        out["fp64_oracle_rel_err_by_config"] = {
+            "c1_radial_256": out["small"]["fp64_oracle_rel_err"],
+            "c2_sense_radial_320": out["fp64_oracle_rel_err"],
+            "c3_batched_spirals": out["batched"]["fp64_oracle_rel_err"],
+            "c4_grad_spiral_256": out["grad"]["fp64_oracle_rel_err"],
+            "c5_kooshball_160": out["kooshball"]["fp64_oracle_rel_err"],
+        }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, ktraj, smaps, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -33,5 +33,9 @@ Parity pins (what checks each function; see ``tests/test_oracle.py``)
   phase, and its adjoint to the unnormalised inverse FFT (``test_nudft_on_grid_is_shifted_fft``,
   1e-12); ``nudft_domega`` against FP64 central differences of ``nudft``.
 * ``nudft.exact_kb_interp``: agreement with the table path to the table-quantisation
-  error, and its derivative against central differences.
+  error (``test_table_interp_matches_exact_kb``).
+* ``nudft.exact_kb_interp_domega``: per-coordinate FP64 central differences of
+  ``exact_kb_interp`` in 1-D/2-D/3-D away from footprint edges (1e-6,
+  ``test_exact_kb_domega_matches_fd``); the directional finite-difference protocol of
+  the differentiation workload against it (``test_directional_fd_protocol_matches_analytic_gradient``).
 """

@@ -99,3 +99,50 @@ def exact_kb_interp(
         out += coef[None, :] * gf[:, idx]
     phase = np.exp(1j * np.sum(om * np.asarray(n_shift, dtype=np.float64)[:, None], axis=0))
     return (out * phase[None, :]).reshape(*lead, om.shape[1])
+
+
+def exact_kb_interp_domega(
+    grid: np.ndarray,
+    omega: np.ndarray,
+    im_size: Sequence[int],
+    numpoints: Sequence[int],
+    alpha: Sequence[float],
+    n_shift: Sequence[float],
+) -> np.ndarray:
+    """d/d omega_{k,m} of ``exact_kb_interp`` (away from footprint edges, where it is
+    smooth): (d, ..., M) complex128.  Per tap the kernel factor of dimension k is
+    replaced by its derivative in omega_k (``kb.kb_deriv`` and the table phase, times
+    K_k / 2 pi), and the shift phase contributes i n_shift_k y."""
+    d = omega.shape[0]
+    grid_size = grid.shape[-d:]
+    lead = grid.shape[:-d]
+    gf = grid.reshape(-1, int(np.prod(grid_size))).astype(np.complex128)
+    om = omega.astype(np.float64)
+    kk = np.asarray(grid_size, dtype=np.float64)
+    tm = om * (kk[:, None] / (2 * np.pi))
+    base = 1 + np.floor(tm - np.asarray(numpoints, dtype=np.float64)[:, None] / 2.0)
+    val = np.zeros((gf.shape[0], om.shape[1]), dtype=np.complex128)
+    der = np.zeros((d, gf.shape[0], om.shape[1]), dtype=np.complex128)
+    strides = [int(np.prod(grid_size[k + 1 :])) for k in range(d)]
+    for offs in np.ndindex(*numpoints):
+        f, df = [], []
+        idx = np.zeros(om.shape[1], dtype=np.int64)
+        for k in range(d):
+            g = base[k] + offs[k]
+            u = tm[k] - g
+            ph = np.exp(-1j * np.pi * (im_size[k] - 1) * u / grid_size[k])
+            v = kb.kb_value(u, numpoints[k], alpha[k])
+            f.append(v * ph)
+            dv = kb.kb_deriv(u, numpoints[k], alpha[k]) + v * (-1j * np.pi * (im_size[k] - 1) / grid_size[k])
+            df.append(dv * ph * (kk[k] / (2 * np.pi)))
+            idx = idx + np.mod(g.astype(np.int64), grid_size[k]) * strides[k]
+        x = gf[:, idx]
+        prod_all = np.prod(f, axis=0)
+        val += prod_all[None, :] * x
+        for k in range(d):
+            others = np.prod([f[j] for j in range(d) if j != k], axis=0) if d > 1 else 1.0
+            der[k] += (df[k] * others)[None, :] * x
+    shift = np.asarray(n_shift, dtype=np.float64)
+    phase = np.exp(1j * np.sum(om * shift[:, None], axis=0))
+    out = (der + 1j * shift[:, None, None] * val[None]) * phase[None, None, :]
+    return out.reshape(d, *lead, om.shape[1])

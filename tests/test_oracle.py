@@ -202,6 +202,67 @@ def test_kb_derivative_matches_fd():
     assert np.abs(cd - t[2:-3]).max() / np.abs(t).max() < 1e-4
 
 
+def _off_edge(om, grid, numpoints, margin):
+    """Samples whose footprint edges are at least `margin` grid units from a grid line
+    in every dimension (the exact-KB operator is smooth there)."""
+    tm = om * (np.asarray(grid, dtype=np.float64)[:, None] / (2 * np.pi)) - numpoints / 2.0
+    return np.all(np.abs(tm - np.rint(tm)) > margin, axis=0)
+
+
+@pytest.mark.parametrize("im_size", [(9,), (8, 7), (5, 6, 4)])
+def test_exact_kb_domega_matches_fd(im_size):
+    rng = np.random.default_rng(11)
+    d = len(im_size)
+    grid = tuple(2 * n for n in im_size)
+    g = rng.standard_normal((2,) + grid) + 1j * rng.standard_normal((2,) + grid)
+    om = rng.uniform(-np.pi, np.pi, (d, 40))
+    om = om[:, _off_edge(om, grid, 6, 1e-3)]
+    alpha, J, shift = [2.34 * 6] * d, [6] * d, [n // 2 for n in im_size]
+    an = nudft.exact_kb_interp_domega(g, om, im_size, J, alpha, shift)
+    h = 1e-7
+    for k in range(d):
+        e = np.zeros_like(om)
+        e[k] = h
+        fd = (nudft.exact_kb_interp(g, om + e, im_size, J, alpha, shift)
+              - nudft.exact_kb_interp(g, om - e, im_size, J, alpha, shift)) / (2 * h)
+        assert np.linalg.norm(an[k] - fd) / np.linalg.norm(fd) < 1e-6
+
+
+def test_directional_fd_protocol_matches_analytic_gradient():
+    """The trajectory-gradient protocol of the differentiation workload, at a reduced
+    size: loss ||A(x0, omega) - y||^2 with the exact-KB operator, FP64 central
+    differences (step 1e-6) along random unit directions supported on 256 samples away
+    from footprint edges, against the analytic gradient 2 Re sum_c conj(r) dA/domega.
+    A spiral interleave starts at omega = 0, exactly on a footprint edge, so the edge
+    filter is part of the protocol."""
+    rng = np.random.default_rng(12)
+    n, coils = 32, 3
+    t = np.arange(96) / 96
+    om = np.concatenate([np.stack([np.pi * t**1.5 * np.sin(2 * np.pi * 3 * t + l), np.pi * t**1.5 *
+                                   np.cos(2 * np.pi * 3 * t + l)]) for l in range(8)], axis=1)
+    grid = (2 * n, 2 * n)
+    G = rng.standard_normal((coils,) + grid) + 1j * rng.standard_normal((coils,) + grid)
+    y = rng.standard_normal((coils, om.shape[1])) + 1j * rng.standard_normal((coils, om.shape[1]))
+    args = ((n, n), (6, 6), [2.34 * 6] * 2, (n // 2, n // 2))
+    r = nudft.exact_kb_interp(G, om, *args) - y
+    grad = 2 * np.sum(np.real(np.conj(r)[None] * nudft.exact_kb_interp_domega(G, om, *args)), axis=1)
+    ok = _off_edge(om, grid, 6, 1e-3)
+    assert not ok[0]  # omega = 0 sits on an edge
+    pool = np.nonzero(ok)[0]
+    h = 1e-6
+    for _ in range(8):
+        sub = rng.choice(pool, 256, replace=False)
+        dirv = rng.standard_normal((2, sub.size))
+        dirv /= np.linalg.norm(dirv)
+        o = om.copy()
+        o[:, sub] += h * dirv
+        lp = np.sum(np.abs(nudft.exact_kb_interp(G, o, *args) - y) ** 2)
+        o[:, sub] -= 2 * h * dirv
+        lm = np.sum(np.abs(nudft.exact_kb_interp(G, o, *args) - y) ** 2)
+        fd = (lp - lm) / (2 * h)
+        assert abs(np.sum(grad[:, sub] * dirv) - fd) <= 1e-5 * abs(fd)
+
+
 def _dense_interp_matrix(plan, om, grid):
     """Columns of the forward interpolation applied to unit grids: (M, prod(grid))."""
     g = int(np.prod(grid))
