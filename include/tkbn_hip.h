@@ -157,6 +157,27 @@ TKBN_API int tkbn_interp_adj(const tkbn_interp_desc* desc, const void* kdata, co
                     int32_t mode, void* workspace, int64_t workspace_bytes, void* stream);
 
 /*
+ * tkbn_interp_adj without a workspace argument and without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a static device array of
+ * tkbn_interp_adj_noalloc_max_bytes() bytes (128 MiB) that each GPU the library runs
+ * on holds until the process ends; a problem whose tkbn_interp_adj_workspace_bytes
+ * exceeds it returns TKBN_ERR_WORKSPACE before any GPU work.  Results are those of
+ * tkbn_interp_adj (same kernels).  All calls on one GPU share the array, which keeps
+ * the last call's trajectory plan (reused by a next call with bitwise the same omega,
+ * tables and geometry, rebuilt otherwise): calls that overlap on one GPU (different
+ * streams, or several host threads) race and must be ordered by the caller.
+ * desc->flags must be 0.  The first call on a device looks up the array's address and
+ * clears its plan-cache header with a kernel on `stream`; make that first call
+ * outside graph capture (a captured first call still gives correct results, but every
+ * replay then rebuilds the plan).
+ */
+TKBN_API int64_t tkbn_interp_adj_noalloc_max_bytes(void);
+TKBN_API int tkbn_interp_adj_noalloc(const tkbn_interp_desc* desc, const void* kdata, const void* omega, void* grid,
+                                     int32_t mode, void* stream);
+
+/*
  * Trajectory gradient of a table interpolation:
  *   grad_omega[d, m] = Re sum_{b,c} conj(grad_kdata[b, c, m]) * d kdata[b, c, m] / d omega[d, m]
  * with kdata = table_interp(grid, omega).  For table_interp_adjoint's gradient pass the

@@ -239,3 +239,24 @@ def test_workspace_size_queries_are_host_only_and_positive(ndim, rows):
     assert lib.tkbn_interp_fwd_workspace_bytes(ctypes.byref(d)) >= 0
     d.flags = 2
     assert lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), 0) == -1  # unknown flag
+
+
+def test_noalloc_adjoint_refuses_oversized_and_flagged_calls_on_the_host():
+    """tkbn_interp_adj_noalloc rejects a problem whose workspace exceeds its static
+    scratch, and any flags, before touching the GPU (pointers are never dereferenced)."""
+    lib = _lib.load()
+    cap = lib.tkbn_interp_adj_noalloc_max_bytes()
+    assert cap == 128 << 20
+    d = _desc(ndim=3)
+    for i in range(3):
+        d.grid_size[i] = 320
+    d.coils, d.klength = 8, 1 << 24
+    for mode in (_lib.TKBN_ADJ_ATOMIC, _lib.TKBN_ADJ_DETERMINISTIC):
+        assert lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), mode) > cap
+        assert lib.tkbn_interp_adj_noalloc(ctypes.byref(d), 1, 1, 1, mode, None) == _lib.TKBN_ERR_WORKSPACE
+        assert b"static scratch" in lib.tkbn_last_error()
+    d.klength = 1000
+    d.flags = _lib.TKBN_PLAN_REUSE
+    assert lib.tkbn_interp_adj_noalloc(ctypes.byref(d), 1, 1, 1, 0, None) == _lib.TKBN_ERR_INVALID
+    assert b"flags" in lib.tkbn_last_error()
+    assert lib.tkbn_interp_adj_noalloc(None, 1, 1, 1, 0, None) == _lib.TKBN_ERR_INVALID

@@ -64,6 +64,11 @@ _SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_i32, _c_vp, _c_i64, _c_vp],
     ),
+    "tkbn_interp_adj_noalloc_max_bytes": (_c_i64, []),
+    "tkbn_interp_adj_noalloc": (
+        ctypes.c_int,
+        [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_i32, _c_vp],
+    ),
     "tkbn_interp_domega": (ctypes.c_int, [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "tkbn_profile_next_kernel": (ctypes.c_int, [_c_vp, _c_vp]),
     "tkbn_last_interp_kernel": (ctypes.c_char_p, []),

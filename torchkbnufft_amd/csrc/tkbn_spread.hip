@@ -23,6 +23,7 @@
 // the result does not depend on the order samples arrive in) and never splits tiles:
 // the output is bitwise repeatable.  The fixed-point scale is 2^e with
 // e = 60 - ceil(log2(2 * max|kdata| * candidates)), so no tile sum can overflow.
+#include <mutex>
 #include <type_traits>
 
 #include "tkbn_geom.h"
@@ -1850,9 +1851,68 @@ int interp_fwd_tiled(const tkbn_interp_desc* d, const void* grid, const void* om
 
 }  // namespace tkbn
 
+// ---- adjoint without device allocation -------------------------------------------
+// tkbn_interp_adj_noalloc keeps its workspace (plan-cache header, trajectory copy,
+// plan) in `scratch`, a static device array that every GPU the library runs on holds
+// from code-object load until the process ends.  In host code `scratch` names a
+// host-side stand-in, never the device memory: the device address comes from
+// hipGetSymbolAddress, once per device with that device current, and only that
+// address is passed to kernels.
+namespace tkbn {
+constexpr int64_t kNoAllocScratchBytes = 128ll << 20;
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace tkbn
+
+static __device__ __align__(256) unsigned char scratch[tkbn::kNoAllocScratchBytes];
+
+namespace tkbn {
+// Device address of the current device's `scratch`; the first call on a device zeroes
+// its plan-cache header (a zero header never matches, so the first call builds).
+static int noalloc_scratch(hipStream_t s, void** out) {
+  int dev = -1;
+  TKBN_HIP_CHECK(hipGetDevice(&dev));
+  TKBN_REQUIRE(dev >= 0 && dev < kMaxDevices, TKBN_ERR_UNSUPPORTED, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    TKBN_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TKBN_REQUIRE(p != nullptr, TKBN_ERR_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+    const int st = launch_zero_fill(static_cast<uint32_t*>(p), kCacheHdrBytes / 4, s);
+    if (st != TKBN_OK) return st;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TKBN_OK;
+}
+}  // namespace tkbn
+
 using namespace tkbn;
 
 extern "C" {
+
+int64_t tkbn_interp_adj_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+int tkbn_interp_adj_noalloc(const tkbn_interp_desc* desc, const void* kdata, const void* omega, void* grid,
+                            int32_t mode, void* stream) {
+  int st = validate_desc(desc);
+  if (st != TKBN_OK) return st;
+  TKBN_REQUIRE(mode == TKBN_ADJ_ATOMIC || mode == TKBN_ADJ_DETERMINISTIC, TKBN_ERR_INVALID, "bad adjoint mode %d", mode);
+  TKBN_REQUIRE(desc->flags == 0, TKBN_ERR_INVALID,
+               "tkbn_interp_adj_noalloc takes no flags (its scratch may hold another call's plan)");
+  if (desc->batch == 0 || desc->coils == 0) return TKBN_OK;
+  const int64_t need = tkbn_interp_adj_workspace_bytes(desc, mode);
+  TKBN_REQUIRE(need >= 0 && need <= kNoAllocScratchBytes, TKBN_ERR_WORKSPACE,
+               "tkbn_interp_adj_noalloc: this problem needs %lld bytes of workspace, more than the %lld-byte "
+               "static scratch; use tkbn_interp_adj with a caller workspace",
+               (long long)need, (long long)kNoAllocScratchBytes);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  void* ws = nullptr;
+  st = noalloc_scratch(s, &ws);
+  if (st != TKBN_OK) return st;
+  return tkbn_interp_adj(desc, kdata, omega, grid, mode, ws, kNoAllocScratchBytes, stream);
+}
 
 int tkbn_profile_next_kernel(void* start_event, void* stop_event) {
   g_profile.start = static_cast<hipEvent_t>(start_event);
