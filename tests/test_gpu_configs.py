@@ -230,9 +230,14 @@ def test_c4_ktraj_gradient_vs_exact_kb_finite_differences():
         dirv /= np.linalg.norm(dirv)
         fd = (loss_on(om0[:, sub] + h * dirv, sub) - loss_on(om0[:, sub] - h * dirv, sub)) / (2 * h)
         an = float(np.sum(gk[:, sub] * dirv))
-        errs.append(abs(an - fd) / max(abs(fd), 1e-30))
-    print(f"[C4 ktraj] directional rel err: median {np.median(errs):.2e} max {max(errs):.2e}")
-    assert np.median(errs) < 5e-3 and max(errs) < 3e-2
+        # |an - fd| against the RMS size of a directional derivative on this support,
+        # |grad on support| / sqrt(coordinates): a ratio to fd itself is unbounded when
+        # a random direction is nearly orthogonal to the gradient
+        errs.append(abs(an - fd) * np.sqrt(dirv.size) / np.linalg.norm(gk[:, sub]))
+    print(f"[C4 ktraj] directional err (RMS-normalised): median {np.median(errs):.2e} max {max(errs):.2e}")
+    # the table path's gradient differs from the exact-KB one by the table quantisation
+    # (a few 1e-3, a little more than its values' 7e-4)
+    assert np.median(errs) < 6e-3 and max(errs) < 2e-2
 
     r0 = nudft.exact_kb_interp(G0, om0, (n, n), (6, 6), alpha, (n // 2, n // 2)) - yy
     # the whole trajectory gradient against the analytic exact-KB one (off the edges)
@@ -240,7 +245,7 @@ def test_c4_ktraj_gradient_vs_exact_kb_finite_differences():
     g_an = 2 * np.sum(np.real(np.conj(r0)[None] * dA), axis=1)
     e_all = float(np.linalg.norm(gk[:, pool] - g_an[:, pool]) / np.linalg.norm(g_an[:, pool]))
     print(f"[C4 ktraj] rel L2 vs analytic exact-KB gradient over {pool.size} samples: {e_all:.2e}")
-    assert e_all < 5e-3
+    assert e_all < 6e-3
     del dA
     xerrs = []
     for _ in range(4):
