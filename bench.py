@@ -91,7 +91,7 @@ def time_stages(ops, x, ktraj, smaps, iters: int):
     lib = _lib.load()
     fused = _fft.fused_supported(x.dtype, (N, N), (K, K))
     names = ["fft_fwd", "interp", "spread", "fft_adj"]
-    kern = ["gather_coils_kernel", "spread_coil_regs_kernel"]
+    kern = ["gather_sep_kernel", "spread_sep_kernel"]
     acc = {n: 0.0 for n in names + kern}
 
     def run(ev, kev):
@@ -656,7 +656,7 @@ def main() -> int:
     sb = stage_bytes(F, M)
     stages, kernels = time_stages((fw, ad), x, ktraj, smaps, args.stage_iters)
     # algorithmic bytes of one launch of each binned kernel = its stage's bytes
-    kbytes = {"gather_coils_kernel": sb["interp"], "spread_coil_regs_kernel": sb["spread"]}
+    kbytes = {"gather_sep_kernel": sb["interp"], "spread_sep_kernel": sb["spread"]}
     dom = max(kernels, key=kernels.get)
     achieved = kbytes[dom] / (kernels[dom] * 1e-6) / 1e9
     pair_bytes = sum(sb.values())

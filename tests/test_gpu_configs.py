@@ -54,7 +54,7 @@ def test_c2_forward_all_rows_match_oracle(c2):
     print("[case] C2 forward, 4 x 16 coils, all rows", flush=True)
     fw = tkbn.KbNufft(im_size=(320, 320), dtype=torch.complex64).to(DEV)
     kd = fw(x.to(DEV), ktraj.to(DEV), smaps=smaps.to(DEV))
-    assert _last_kernel() == "gather_coils_kernel"
+    assert _last_kernel() == "gather_sep_kernel"
     errs = _row_errors(kd, kd_ref)
     print(f"[C2 fwd] rel L2 all {rel_l2(kd, kd_ref):.2e}, worst row {max(errs):.2e}")
     assert len(errs) == 64 and max(errs) < C64_TOL
@@ -68,7 +68,7 @@ def test_c2_adjoint_all_rows_match_oracle(c2, deterministic):
     y = kd_ref.to(DEV)
     args = (kt,) + _module_args((320, 320)) + ((640, 640),)
     grid = _interp.table_interp_adjoint(y, *args, deterministic=deterministic)
-    want = "spread_tiles_kernel<fixed,deterministic>" if deterministic else "spread_coil_regs_kernel"
+    want = "spread_tiles_kernel<fixed,deterministic>" if deterministic else "spread_sep_kernel"
     assert _last_kernel() == want
     grid_ref = tp.kb_table_interp_adjoint(plan, kd_ref, ktraj)
     errs = _row_errors(grid, grid_ref)
@@ -103,7 +103,7 @@ def test_multicoil_split_tiles_match_oracle(rows):
     y = W.cplx((1, rows, om.shape[1]), gen)
     args = (om.to(DEV),) + _module_args((80, 80)) + (grid,)
     out = _interp.table_interp_adjoint(y.to(DEV), *args, deterministic=False)
-    assert _last_kernel() == "spread_coil_regs_kernel"
+    assert _last_kernel() == "spread_sep_kernel"
     plan = tp.Plan((80, 80), grid, dtype=torch.complex64)
     sel = list(range(0, rows, 7)) + [rows - 1]
     ref = tp.kb_table_interp_adjoint(plan, y[:, sel], om)
@@ -112,7 +112,7 @@ def test_multicoil_split_tiles_match_oracle(rows):
     x = W.cplx((1, rows) + grid, gen)
     fw = tkbn.KbInterp(im_size=(80, 80), grid_size=grid, dtype=torch.complex64).to(DEV)
     kd = fw(x.to(DEV), om.to(DEV))
-    assert _last_kernel() == "gather_coils_kernel"
+    assert _last_kernel() == "gather_sep_kernel"
     errs = _row_errors(kd[:, sel], tp.kb_table_interp(plan, x[:, sel], om))
     assert max(errs) < C64_TOL, errs
 

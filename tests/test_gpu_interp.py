@@ -13,6 +13,7 @@ import torchkbnufft_amd as tkbn
 from conftest import create_input_plus_noise, create_ktraj, golden, rel_l2
 from oracle import nudft
 from oracle import table_path as tp
+from torchkbnufft_amd import _lib
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda")
@@ -367,3 +368,64 @@ def test_plan_cache_key_covers_tile_shape():
         assert rel_l2(f, fw(x, om)) < 1e-6, rows
         _interp.clear_plan_cache()
         assert rel_l2(a, ad(y, om)) < 1e-6, rows
+
+
+def _rounding_taps(om: np.ndarray, K: int, J: int = 6, L: int = 1024) -> np.ndarray:
+    """Samples whose float32 tap indices (the reference's round((tm - g) L)) are not all
+    i_0 - j L along this dimension (the multi-coil kernels' complex-weight entries)."""
+    om = om.astype(np.float32)
+    gam = np.float32(np.float32(1) / np.float32(K)) * np.float32(6.283185307179586)
+    tm = (om / gam).astype(np.float32)
+    base = np.floor(tm - np.float32(J / 2)).astype(np.float32) + np.float32(1)
+    i0 = np.rint((tm - base).astype(np.float32) * np.float32(L))
+    bad = np.zeros(om.shape, bool)
+    for j in range(1, J):
+        bad |= np.rint((tm - (base + np.float32(j))).astype(np.float32) * np.float32(L)) != i0 - j * L
+    return bad
+
+
+@pytest.mark.parametrize("table", ["reference", "perturbed"])
+def test_multicoil_exception_entries_match_oracle(table):
+    """The multi-coil kernels apply real tap weights on a phase-rotated grid where the
+    table is the reference's KB(u) exp(-i pi (N-1) u / K) and every tap index is i_0 - j L;
+    other entries take complex weights.  Samples near k = 0 hit the float32 rounding
+    case, and a table with an extra non-linear phase makes every entry complex: both
+    against the oracle, forward and adjoint, 40 coil rows."""
+    print(f"[case] multicoil exception entries table={table}", flush=True)
+    gen = torch.Generator().manual_seed(43)
+    im, grid, rows = (64, 64), (128, 128), 40
+    # rounding cases are rare (about 1 in 10^4 near k = 0): take them from a scan of
+    # 4e5 near-centre values, paired with random second coordinates
+    near = ((torch.rand(400000, generator=gen) - 0.5) * 0.4).numpy().astype(np.float32)
+    hits = torch.from_numpy(near[_rounding_taps(near, 128)][:48])
+    assert hits.numel() >= 16
+    odd = torch.stack([hits, (torch.rand(hits.numel(), generator=gen) - 0.5) * 6.0])
+    om = torch.cat([create_ktraj(2, 3000, gen, torch.float32), odd, odd.flip(0)], dim=1)
+    bad = _rounding_taps(om[0].numpy(), 128) | _rounding_taps(om[1].numpy(), 128)
+    assert bad.sum() >= 2 * hits.numel()
+    ad = tkbn.KbInterpAdjoint(im_size=im, grid_size=grid, dtype=torch.complex64).to(DEV)
+    plan = tp.Plan(im, grid, dtype=torch.complex64)
+    tabs = [t.cpu() for t in plan.tables]
+    if table == "perturbed":
+        m = torch.arange(tabs[0].numel(), dtype=torch.float64)
+        tabs = [(t.to(torch.complex128) * torch.exp(1j * 0.3 * torch.sin(m / 97.0))).to(torch.complex64) for t in tabs]
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    x = gi.cplx((1, rows) + grid, gen)
+    y = gi.cplx((1, rows, om.shape[1]), gen)
+    dt = [t.to(DEV) for t in tabs]
+    args = (ad.n_shift, ad.numpoints, ad.table_oversamp, ad.offsets)
+    kd = _interp.table_interp(x.to(DEV), om.to(DEV), dt, *args)
+    assert _lib.load().tkbn_last_interp_kernel().decode() == "gather_sep_kernel"
+    gr = _interp.table_interp_adjoint(y.to(DEV), om.to(DEV), dt, *args, grid, deterministic=False)
+    assert _lib.load().tkbn_last_interp_kernel().decode() == "spread_sep_kernel"
+    ns, J, L, off = plan.t_n_shift, plan.t_numpoints, plan.t_table_oversamp, plan.offsets
+    sel = [0, 17, rows - 1]
+    ref_f = tp.table_interp(x[:, sel], om, tabs, ns, J, L, off)
+    ref_a = tp.table_interp_adjoint(y[:, sel], om, tabs, ns, J, L, off, plan.t_grid_size)
+    e_f, e_a = rel_l2(kd[:, sel].cpu(), ref_f), rel_l2(gr[:, sel].cpu(), ref_a)
+    # the rounding-case samples alone
+    idx = torch.nonzero(torch.as_tensor(bad)).flatten()
+    e_fb = rel_l2(kd[:, sel][..., idx].cpu(), ref_f[..., idx])
+    print(f"[exceptions {table}] {int(bad.sum())} rounding samples; fwd {e_f:.2e} (those {e_fb:.2e}) adj {e_a:.2e}")
+    assert e_f < C64_TOL and e_a < C64_TOL and e_fb < C64_TOL

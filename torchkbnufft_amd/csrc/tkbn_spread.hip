@@ -521,18 +521,12 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
-// multi-coil spread with register accumulators (2-D, complex64, J <= 6)
-//   One group (8 waves) per (16 x 16 tile, split, chunk of 64 coil rows); lanes are
-//   coil rows.  Wave w owns tile rows 2w and 2w+1: their 2 x 16 cells live in its
-//   registers, so every cell has exactly one writer and no atomics are needed.
-//   The plan stores, for every (sample, tile) entry, a record with the sample's tap
-//   weights (phase folded into dimension 0, both conjugated) and the footprint's first
-//   row r0 and column c0 relative to the tile, and keeps each tile's entries close to
-//   trajectory order.  The kernel stages 64 entries at a time (records, and the
-//   entries' data with lanes = entries, so the samples of one spoke load coalesced),
-//   then walks the staged entries c0 value by c0 value (one ballot each): inside a
-//   group the column offsets are compile-time constants, so every accumulator index
-//   is fixed and the taps are plain FMAs.
+// multi-coil spread geometry (2-D, complex64, J <= 6; the kernel is spread_sep_kernel)
+//   One group (8 waves) per (16 x 16 tile, split, chunk of 32 coil rows); half-wave h
+//   of wave w owns tile row 2w + h of its coil row in registers, so every cell has
+//   exactly one writer and no atomics are needed.  Entries (sample, tile) are staged
+//   64 at a time and walked c0 value by c0 value (one ballot each): inside a group the
+//   column offsets are compile-time constants, so every accumulator index is fixed.
 //   Footprints are contiguous inside a tile when every grid extent is at least
 //   tile + J - 1 cells (a wrapped footprint then never meets one tile twice);
 //   smaller grids use the sample-lane kernel.
@@ -556,15 +550,6 @@ constexpr int kRegHotTarget = 384;
 __host__ __device__ inline int reg_splits(int v) {
   return v <= kRegMaxUnsplit ? 1 : (v + kRegHotTarget - 1) / kRegHotTarget;
 }
-
-struct CoilRec {
-  float2 w0[6];  // conj(phase * w0[j]) along dimension 0
-  float2 w1[6];  // conj(w1[j]) along dimension 1
-  int r0;        // first footprint row relative to the tile
-  int c0;        // first footprint column relative to the tile
-  int pad[2];
-};
-static_assert(sizeof(CoilRec) == 112, "plan record is 112 bytes");
 
 // Tiles met along one dimension by the footprint bm .. bm+J-1 (mod K), in footprint
 // order, with the footprint's first cell relative to each tile's origin.  The runs of
@@ -616,322 +601,16 @@ __device__ __forceinline__ void coil_sample_tiles(const Geom<float>& g, const Ti
   }
 }
 
-__global__ void __launch_bounds__(256)
-    coil_bin_count_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
-                          int* __restrict__ ws, const CacheHdr* cache) {
-  if (cache_hit(cache)) return;  // cached plan reused
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = m < g.M;
-  int tl[2][3], off[2][3], n[2];
-  coil_sample_tiles(g, tg, omega, m, live, tl, off, n);
-  int* count = ws + tg.o_count;
-  coil_for_each_entry(tg, tl, off, n, live,
-                      [&](int key, int, int, bool valid) { wave_aggregated_add(count, key, valid); });
-}
-
-__global__ void __launch_bounds__(256)
-    coil_bin_scatter_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
-                            int* __restrict__ ws, const CacheHdr* cache) {
-  if (cache_hit(cache)) return;  // cached plan reused
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = m < g.M;
-  int tl[2][3], off[2][3], n[2];
-  coil_sample_tiles(g, tg, omega, m, live, tl, off, n);
-  CoilRec rec;
-  if (live) {
-    float2 w[6];
-    int mdx[6];
-    const float theta = sample_theta<float, 2>(g, omega, m);
-    const float2 ph = make_float2(cosf(theta), sinf(theta));
-    sample_taps_md<float, 6>(g, 0, omega[m], w, mdx);
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const float2 pw = cmul(ph, w[j]);
-      rec.w0[j] = make_float2(pw.x, -pw.y);
-    }
-    sample_taps_md<float, 6>(g, 1, omega[g.M + m], w, mdx);
-#pragma unroll
-    for (int j = 0; j < 6; ++j) rec.w1[j] = make_float2(w[j].x, -w[j].y);
-  }
-  rec.pad[0] = rec.pad[1] = 0;
-  int* plan = ws;
-  CoilRec* recs = reinterpret_cast<CoilRec*>(reinterpret_cast<char*>(ws) + tg.o_rec_bytes);
-  // Positions come from the tile's cursor (its first key), not the (tile, c0) key's:
-  // a tile's entries then keep trajectory order as far as the atomics preserve it, so
-  // the samples of one spoke crossing the tile sit together and the kernel's loads of
-  // their data share cache lines.
-  coil_for_each_entry(tg, tl, off, n, live, [&](int key, int a, int c, bool valid) {
-    const int tkey = valid ? key - (off[1][c] - kRegMinC0) : 0;
-    const int pos = wave_aggregated_inc(plan + tg.o_cursor, tkey, valid);
-    if (valid) {
-      plan[tg.o_perm + pos] = m;
-      rec.r0 = off[0][a];
-      rec.c0 = off[1][c];
-      recs[pos] = rec;
-    }
-  });
-}
-
-// Scan of the (tile, c0) counts: per-tile totals are scanned across the block, then
-// each tile's keys are offset locally.  The totals also give the splits and work list.
-__global__ void __launch_bounds__(1024) coil_plan_scan_kernel(const TileGeom tg, int* __restrict__ ws,
-                                                             const CacheHdr* cache) {
-  if (cache_hit(cache)) return;  // cached plan reused
-  __shared__ int sbuf[1024];
-  int* plan = ws;
-  const int* count = plan + tg.o_count;
-  int* start = plan + tg.o_start;
-  int* cursor = plan + tg.o_cursor;
-  int* wstart = plan + tg.o_wstart;
-  int* work = plan + tg.o_work;
-  const int NB = tg.NB;
-  // 1. tile totals (work[0..NB) as scratch), scanned into cursor[0..NB] (scratch)
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
-    const int* c = count + (int64_t)t * kRegKeys;
-    int tot = 0;
-#pragma unroll
-    for (int k = 0; k < kRegKeys; ++k) tot += c[k];
-    work[t] = tot;
-  }
-  __syncthreads();
-  block_exclusive_scan(work, cursor, NB, sbuf);
-  // 2. key starts: tile start + prefix over the tile's keys
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
-    const int* c = count + (int64_t)t * kRegKeys;
-    int run = cursor[t];
-    int* st = start + (int64_t)t * kRegKeys;
-#pragma unroll
-    for (int k = 0; k < kRegKeys; ++k) {
-      st[k] = run;
-      run += c[k];
-    }
-  }
-  if (threadIdx.x == 0) start[tg.nkeys] = cursor[NB];
-  __syncthreads();
-  // 3. cursors, splits and the work list: the pieces of split tiles first
-  for (int64_t i = threadIdx.x; i < tg.nkeys; i += blockDim.x) cursor[i] = start[i];
-  int* hot = plan + tg.o_hot;    // NB + 1: pieces of split tiles, then their scan
-  int* cold = plan + tg.o_cold;  // NB + 1: 1 for unsplit tiles, then their scan
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
-    const int v = start[(int64_t)(t + 1) * kRegKeys] - start[(int64_t)t * kRegKeys];
-    const int ns = reg_splits(v);
-    work[t] = ns > 1 ? ns : 0;
-    wstart[t] = ns > 1 ? 0 : 1;
-  }
-  __syncthreads();
-  block_exclusive_scan(work, hot, NB, sbuf);
-  block_exclusive_scan(wstart, cold, NB, sbuf);
-  const int nhot = hot[NB];
-  const int cap = (int)tg.maxwork;
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
-    const int ns = hot[t + 1] - hot[t];
-    wstart[t] = ns > 0 ? hot[t] : nhot + cold[t];
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
-    const int ns = max(hot[t + 1] - hot[t], 1);
-    const int a = wstart[t], e = min(a + ns, cap);
-    for (int w = a; w < e; ++w) work[w] = t;
-  }
-  if (threadIdx.x == 0) plan[tg.o_nwork] = min(nhot + cold[NB], cap);
-}
-
-// zeroes the tiles whose pieces add into the grid
-__global__ void __launch_bounds__(256)
-    coil_zero_hot_kernel(const Geom<float> g, const TileGeom tg, const int* __restrict__ ws, float2* __restrict__ grid) {
-  const int t = blockIdx.x;
-  const int* start = ws + tg.o_start;
-  const int v = start[(int64_t)(t + 1) * kRegKeys] - start[(int64_t)t * kRegKeys];
-  if (reg_splits(v) <= 1) return;
-  const int64_t nrows = g.B * g.C;
-  const int64_t r0 = (int64_t)blockIdx.y * 64;
-  const int64_t r1 = r0 + 64 < nrows ? r0 + 64 : nrows;
-  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kRegT0;
-  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kRegT1;
-  for (int cell = threadIdx.x; cell < kRegT0 * kRegT1; cell += blockDim.x) {
-    const int k0 = o0 + cell / kRegT1, k1 = o1 + cell % kRegT1;
-    if (k0 >= g.K[0] || k1 >= g.K[1]) continue;
-    for (int64_t r = r0; r < r1; ++r) grid[r * g.G + (int64_t)k0 * g.stride[0] + k1] = make_float2(0.f, 0.f);
-  }
-}
-
-// acc[C0 + j] += u * w1[j] for the footprint columns that fall inside the tile
-template <int C0>
-__device__ __forceinline__ void reg_add_row(float2 (&acc)[kRegT1], const float2 u, const float2 (&w1)[6]) {
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    if (C0 + j >= 0 && C0 + j < kRegT1) {
-      acc[C0 + j].x = fmaf(u.x, w1[j].x, acc[C0 + j].x);
-      acc[C0 + j].x = fmaf(-u.y, w1[j].y, acc[C0 + j].x);
-      acc[C0 + j].y = fmaf(u.x, w1[j].y, acc[C0 + j].y);
-      acc[C0 + j].y = fmaf(u.y, w1[j].x, acc[C0 + j].y);
-    }
-  }
-}
-
-// one entry with footprint column C0 for the lane's tile row (footprint row j0 of the
-// entry; outside 0 .. J0-1 the lane adds nothing).  The weights come from the staged
-// record in LDS: w1 at a wave-uniform address, w0 per half-wave; every operand is
-// read before the first use (one LDS wait per entry).
-template <int C0>
-__device__ __forceinline__ void reg_entry(float2 (&acc)[kRegT1], const CoilRec* rec, const float2 v, int j0, int J0) {
-  float2 w1[6];
-#pragma unroll
-  for (int j = 0; j < 6; ++j) w1[j] = rec->w1[j];
-  const float2 w0 = rec->w0[min(max(j0, 0), 5)];
-  if (j0 >= 0 && j0 < J0) {
-    float2 u;
-    u.x = fmaf(v.x, w0.x, -v.y * w0.y);
-    u.y = fmaf(v.x, w0.y, v.y * w0.x);
-    reg_add_row<C0>(acc, u, w1);
-  }
-}
-
-__global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_eu(6)))
-    spread_coil_regs_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ kdata,
-                            const int* __restrict__ ws, float2* __restrict__ grid, int nchunk) {
-  __shared__ __attribute__((aligned(16))) float2 vals[kRegBatch * kRegPitch];
-  __shared__ __attribute__((aligned(16))) CoilRec brec[kRegBatch];  // the batch's records
-  // wave-uniform in a scalar register, so the hit tests below are scalar branches
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int coil = lane & (kRegCoils - 1), hh = lane / kRegCoils;  // lane = (tile row of the wave, coil)
-
-  const int* plan = ws;
-  // the chunks of one work item are consecutive, so hot pieces of every chunk start first
-  const int64_t wi = blockIdx.x / nchunk;
-  const int chunk = (int)(blockIdx.x - wi * nchunk);
-  if (wi >= plan[tg.o_nwork]) return;
-  const int* wstart = plan + tg.o_wstart;
-  const int t = plan[tg.o_work + wi];
-  const int split = (int)(wi - wstart[t]);
-  const int64_t nrows = g.B * g.C;
-  const int64_t rbase = (int64_t)chunk * kRegCoils;
-  const int ncc = (int)((rbase + kRegCoils < nrows ? rbase + kRegCoils : nrows) - rbase);
-  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kRegT0;
-  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kRegT1;
-  const int cstart = plan[tg.o_start + t * kRegKeys];
-  const int V = plan[tg.o_start + (t + 1) * kRegKeys] - cstart;
-  const int nsplit = reg_splits(V);
-  const int i0 = nsplit > 1 ? split * kRegHotTarget : 0;
-  const int i1 = nsplit > 1 ? min(i0 + kRegHotTarget, V) : V;
-  const int* perm = plan + tg.o_perm + cstart;
-  const CoilRec* recs = reinterpret_cast<const CoilRec*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) + cstart;
-  const int J0 = g.J[0];
-  const int wrow = wave * kRegRowsPerWave;  // the wave's first tile row
-
-  float2 acc[kRegT1];
-#pragma unroll
-  for (int c = 0; c < kRegT1; ++c) acc[c] = make_float2(0.f, 0.f);
-
-  // Software pipeline: while a batch is processed, the next batch's data and records
-  // are loaded into registers (vector loads: the records' first touch of memory is
-  // never a wait inside the entry loop), and the trajectory indices one batch further.
-  constexpr int kRecF4 = (int)(sizeof(CoilRec) / sizeof(float4));  // 7
-  static_assert(kRegBatch * kRecF4 <= kRegThreads, "one record float4 per thread");
-  constexpr int kPer = kRegCoils / kRegWaves;  // coil rows a thread stages per batch
-  float4 prec = make_float4(0.f, 0.f, 0.f, 0.f);
-  float2 pv[kPer];
-  auto prefetch = [&](int b, int mb) {
-    const int n = min(kRegBatch, i1 - b);
-    if ((int)threadIdx.x < n * kRecF4) prec = reinterpret_cast<const float4*>(recs + b)[threadIdx.x];
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int row = wave * kPer + q;
-      pv[q] = make_float2(0.f, 0.f);
-      if (lane < n && row < ncc) pv[q] = kdata[(rbase + row) * g.M + mb];
-    }
-  };
-  int mnext = 0;
-  if (i0 < i1) {
-    const int m0 = lane < min(kRegBatch, i1 - i0) ? perm[i0 + lane] : 0;
-    prefetch(i0, m0);
-    const int b1 = i0 + kRegBatch;
-    if (b1 < i1 && lane < min(kRegBatch, i1 - b1)) mnext = perm[b1 + lane];
-  }
-  for (int base = i0; base < i1; base += kRegBatch) {
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) vals[lane * kRegPitch + wave * kPer + q] = pv[q];
-    const int nb = min(kRegBatch, i1 - base);
-    if ((int)threadIdx.x < nb * kRecF4) reinterpret_cast<float4*>(brec)[threadIdx.x] = prec;
-    __syncthreads();
-    // entry `lane` of this batch (1000: none)
-    const int r0l = lane < nb ? brec[lane].r0 : 1000, c0l = lane < nb ? brec[lane].c0 : 1000;
-    const int b1 = base + kRegBatch;
-    if (b1 < i1) {
-      prefetch(b1, mnext);
-      const int b2 = b1 + kRegBatch;
-      mnext = (b2 < i1 && lane < min(kRegBatch, i1 - b2)) ? perm[b2 + lane] : 0;
-    }
-    // entries whose footprint rows meet this wave's rows
-    const int jal = wrow - r0l;
-    const unsigned long long hits = __ballot(jal + kRegRowsPerWave > 0 && jal < J0);
-    // walk each c0 value's hits with fixed column offsets
-#define TKBN_REG_GROUP(C0)                                                       \
-  {                                                                              \
-    unsigned long long mk = __ballot(c0l == (C0)) & hits;                        \
-    while (mk) {                                                                 \
-      const int c = __ffsll(mk) - 1;                                             \
-      mk &= mk - 1;                                                              \
-      const int ja = wrow - __builtin_amdgcn_readlane(r0l, c);                   \
-      reg_entry<C0>(acc, brec + c, vals[c * kRegPitch + coil], ja + hh, J0);     \
-    }                                                                            \
-  }
-    TKBN_REG_GROUP(-5) TKBN_REG_GROUP(-4) TKBN_REG_GROUP(-3) TKBN_REG_GROUP(-2) TKBN_REG_GROUP(-1)
-    TKBN_REG_GROUP(0) TKBN_REG_GROUP(1) TKBN_REG_GROUP(2) TKBN_REG_GROUP(3) TKBN_REG_GROUP(4)
-    TKBN_REG_GROUP(5) TKBN_REG_GROUP(6) TKBN_REG_GROUP(7) TKBN_REG_GROUP(8) TKBN_REG_GROUP(9)
-    TKBN_REG_GROUP(10) TKBN_REG_GROUP(11) TKBN_REG_GROUP(12) TKBN_REG_GROUP(13) TKBN_REG_GROUP(14)
-    TKBN_REG_GROUP(15)
-#undef TKBN_REG_GROUP
-    __syncthreads();
-  }
-
-  // Flush: every owned cell inside the grid is written (split tiles were zeroed first
-  // and add).  The rows go out through LDS in rounds of kRound tile rows so that each
-  // store instruction writes 16 consecutive cells (128 B) of every coil row.
-  constexpr int kRound = 4;                          // tile rows per round
-  constexpr int kRoundWaves = kRound / kRegRowsPerWave;  // waves whose rows go out in a round
-  static_assert(kRound * kRegT1 * kRegCoils <= kRegBatch * kRegPitch, "a round fits the staging area");
-  static_assert(kRound % kRegRowsPerWave == 0 && (kRound * kRegT1 * kRegCoils) % kRegThreads == 0, "round shape");
-  for (int round = 0; round < kRegT0 / kRound; ++round) {
-    if (wave / kRoundWaves == round) {
-      const int slot = (wave % kRoundWaves) * kRegRowsPerWave + hh;  // tile row kRound * round + slot
-#pragma unroll
-      for (int c = 0; c < kRegT1; ++c) vals[(slot * kRegT1 + c) * kRegPitch + coil] = acc[c];
-    }
-    lds_barrier();
-#pragma unroll
-    for (int q = 0; q < (kRound * kRegT1 * kRegCoils) / kRegThreads; ++q) {
-      const int e = threadIdx.x + q * kRegThreads;
-      const int c = e & (kRegT1 - 1), cl = (e / kRegT1) & (kRegCoils - 1), slot = e / (kRegT1 * kRegCoils);
-      const int k0 = o0 + kRound * round + slot, k1 = o1 + c;
-      if (cl < ncc && k0 < g.K[0] && k1 < g.K[1]) {
-        const float2 v = vals[(slot * kRegT1 + c) * kRegPitch + cl];
-        float2* dst = grid + (rbase + cl) * g.G + (int64_t)k0 * g.stride[0] + k1;
-        if (nsplit == 1) {
-          *dst = v;
-        } else {
-          unsafeAtomicAdd(&dst->x, v.x);
-          unsafeAtomicAdd(&dst->y, v.y);
-        }
-      }
-    }
-    lds_barrier();  // the stores stay in flight through the next rounds
-  }
-}
-
 // ---------------------------------------------------------------------------
-// multi-coil forward gather (2-D, complex64, J <= 6)
-//   The plan bins every sample once, into the tile of its base cell (the
-//   footprint's first cell), keeping trajectory order inside a tile, and stores a
-//   per-entry record: phase * w0[j], w1[j] and the base cell relative to the tile.
-//   One group (8 waves) per (16 x 16 tile, split, chunk of 16 coil rows) stages the
-//   tile and its J-1 halo (21 x 21 cells, wrapped) in LDS with the coil row fastest.
-//   Each quarter-wave takes one entry (lanes = the chunk's 16 rows), so a tap read by
-//   a wave is four runs of 16 consecutive LDS words and conflict-free.  Short row
-//   chunks allow large tiles: the region is 1.7x the tile's cells (3.7x for 4 x 8
-//   tiles of 64 rows), and each staging round covers 8x the samples, hiding its
-//   latency; the price is one record read per chunk.
+// multi-coil forward gather geometry (2-D, complex64, J <= 6; the kernel is
+// gather_sep_kernel)
+//   The plan bins every sample once, into the tile of its base cell (the footprint's
+//   first cell), keeping trajectory order inside a tile.  One group (8 waves) per
+//   (16 x 16 tile, split, chunk of 16 coil rows) stages the tile and its J-1 halo
+//   (21 x 21 cells, wrapped) in LDS with the coil row fastest.  Each quarter-wave takes
+//   one entry (lanes = the chunk's 16 rows), so a tap read by a wave is four runs of
+//   16 consecutive LDS words and conflict-free.  Short row chunks allow large tiles:
+//   the region is 1.7x the tile's cells, and each staging round covers many samples.
 // ---------------------------------------------------------------------------
 constexpr int kGatT0 = 16, kGatT1 = 16;     // tile edges
 constexpr int kGatR0 = kGatT0 + 5;          // staged region edges (J <= 6)
@@ -943,15 +622,6 @@ constexpr int kGatPitch = kGatRows + 1;     // float2 per staged cell (odd: stag
 constexpr int kGatThreads = 512;
 constexpr int kGatTarget = 256;             // entries per work item
 static_assert(kGatCells * kGatPitch * 8 * 2 <= 160 * 1024, "two groups per CU");
-
-struct GatRec {
-  float2 w0[6];  // phase * w0[j] along dimension 0
-  float2 w1[6];  // w1[j] along dimension 1
-  int l0, l1;    // base cell relative to the tile origin
-  int m;         // sample index
-  int pad;
-};
-static_assert(sizeof(GatRec) == 112, "plan record is 112 bytes");
 
 __device__ __forceinline__ int gat_base_tile(const Geom<float>& g, const TileGeom& tg, const float* om, int m, int* l) {
   int key = 0;
@@ -966,89 +636,6 @@ __device__ __forceinline__ int gat_base_tile(const Geom<float>& g, const TileGeo
   return key;
 }
 
-__global__ void __launch_bounds__(256)
-    gat_bin_count_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega, int* __restrict__ ws,
-                         const CacheHdr* cache) {
-  if (cache_hit(cache)) return;  // cached plan reused
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = m < g.M;
-  int l[2];
-  const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
-  wave_aggregated_add(ws + tg.o_count, key, live);
-}
-
-__global__ void __launch_bounds__(256)
-    gat_bin_scatter_kernel(const Geom<float> g, const TileGeom tg, const float* __restrict__ omega,
-                           int* __restrict__ ws,
-                         const CacheHdr* cache) {
-  if (cache_hit(cache)) return;  // cached plan reused
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = m < g.M;
-  int l[2] = {0, 0};
-  const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
-  const int pos = wave_aggregated_inc(ws + tg.o_cursor, key, live);
-  if (!live) return;
-  GatRec rec;
-  float2 w[6];
-  int mdx[6];
-  const float theta = sample_theta<float, 2>(g, omega, m);
-  const float2 ph = make_float2(cosf(theta), sinf(theta));
-  sample_taps_md<float, 6>(g, 0, omega[m], w, mdx);
-#pragma unroll
-  for (int j = 0; j < 6; ++j) rec.w0[j] = cmul(ph, w[j]);
-  sample_taps_md<float, 6>(g, 1, omega[g.M + m], w, mdx);
-#pragma unroll
-  for (int j = 0; j < 6; ++j) rec.w1[j] = w[j];
-  rec.l0 = l[0];
-  rec.l1 = l[1];
-  rec.m = m;
-  rec.pad = 0;
-  ws[tg.o_perm + pos] = m;
-  reinterpret_cast<GatRec*>(reinterpret_cast<char*>(ws) + tg.o_rec_bytes)[pos] = rec;
-}
-
-// Scan of the gather plan: tiles with more than `target` entries are cut into
-// pieces of `target`, listed first (longest work first); the rest follow, one each.
-__global__ void __launch_bounds__(1024) gat_plan_scan_kernel(const TileGeom tg, int* __restrict__ ws,
-                                                            const CacheHdr* cache) {
-  if (cache_hit(cache)) return;  // cached plan reused
-  __shared__ int sbuf[1024];
-  int* plan = ws;
-  const int* count = plan + tg.o_count;
-  int* start = plan + tg.o_start;
-  int* cursor = plan + tg.o_cursor;
-  int* wstart = plan + tg.o_wstart;
-  int* work = plan + tg.o_work;
-  int* hot = plan + tg.o_hot;
-  int* cold = plan + tg.o_cold;
-  const int NB = tg.NB;
-  block_exclusive_scan(count, start, NB, sbuf);
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
-    cursor[t] = start[t];
-    const int v = count[t];
-    const int ns = v <= tg.target ? 1 : (v + tg.target - 1) / tg.target;
-    work[t] = ns > 1 ? ns : 0;
-    wstart[t] = (ns == 1 && v > 0) ? 1 : 0;  // empty tiles gather nothing: no item
-  }
-  __syncthreads();
-  block_exclusive_scan(work, hot, NB, sbuf);
-  block_exclusive_scan(wstart, cold, NB, sbuf);
-  const int nhot = hot[NB];
-  const int cap = (int)tg.maxwork;
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
-    const int ns = hot[t + 1] - hot[t];
-    wstart[t] = ns > 0 ? hot[t] : nhot + cold[t];
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
-    const int ns = hot[t + 1] - hot[t];
-    const int items = ns > 0 ? ns : cold[t + 1] - cold[t];
-    const int a = wstart[t], e = min(a + items, cap);
-    for (int w = a; w < e; ++w) work[w] = t;
-  }
-  if (threadIdx.x == 0) plan[tg.o_nwork] = min(nhot + cold[NB], cap);
-}
-
 // Work item of group b.  Consecutive groups go to different XCDs (round robin), so
 // group b runs on XCD x = b mod 8 as that XCD's k-th group, k = b / 8.  Items are
 // dealt to the XCDs in chunks of kXcdChunk neighbouring tiles (chunk c -> XCD c mod 8):
@@ -1059,137 +646,6 @@ constexpr int kXcdChunk = 4;
 __device__ __forceinline__ int64_t xcd_work_item(int64_t b) {
   const int64_t x = b % 8, k = b / 8;
   return kXcdChunk * (x + 8 * (k / kXcdChunk)) + k % kXcdChunk;
-}
-
-// One entry: sum over the J0 x J1 footprint of w0[j0] * w1[j1] * region cell, for the
-// lane's coil row.  r holds the record (w0 in r[0..2], w1 in r[3..5], (l0, l1, m) in
-// r[6]).  JF > 0: J0 = J1 = JF at compile time (no per-tap branches, every LDS read
-// of the entry in flight together).
-template <int JF>
-__device__ __forceinline__ float2 gat_entry(const float4 (&r)[7], const float2* region, int row, int J0, int J1) {
-  float2 w0[6], w1[6];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    w0[2 * q] = make_float2(r[q].x, r[q].y);
-    w0[2 * q + 1] = make_float2(r[q].z, r[q].w);
-    w1[2 * q] = make_float2(r[3 + q].x, r[3 + q].y);
-    w1[2 * q + 1] = make_float2(r[3 + q].z, r[3 + q].w);
-  }
-  const int l0 = __float_as_int(r[6].x), l1 = __float_as_int(r[6].y);
-  const float2* L0 = region + (l0 * kGatR1 + l1) * kGatPitch + row;
-  float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-  for (int j0 = 0; j0 < 6; ++j0) {
-    if (JF > 0 ? j0 < JF : j0 < J0) {
-      const float2* L = L0 + j0 * (kGatR1 * kGatPitch);
-      // four independent FMA chains (real and imaginary parts of both products)
-      float2 p = make_float2(0.f, 0.f), q = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int j1 = 0; j1 < 6; ++j1) {
-        if (JF > 0 ? j1 < JF : j1 < J1) {
-          const float2 w = w1[j1];
-          const float2 x = L[j1 * kGatPitch];
-          p.x = fmaf(w.x, x.x, p.x);
-          q.x = fmaf(-w.y, x.y, q.x);
-          p.y = fmaf(w.x, x.y, p.y);
-          q.y = fmaf(w.y, x.x, q.y);
-        }
-      }
-      p.x += q.x;
-      p.y += q.y;
-      const float2 w = w0[j0];
-      acc.x = fmaf(w.x, p.x, acc.x);
-      acc.x = fmaf(-w.y, p.y, acc.x);
-      acc.y = fmaf(w.x, p.y, acc.y);
-      acc.y = fmaf(w.y, p.x, acc.y);
-    }
-  }
-  return acc;
-}
-
-template <int JF>
-__global__ void __launch_bounds__(kGatThreads)
-    gather_coils_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ grid,
-                        const int* ws, float2* kdata, int nchunk) {  // no restrict: keeps the record loads vector loads
-  __shared__ __attribute__((aligned(16))) float2 region[kGatCells * kGatPitch];
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const int half = lane / kGatRows, row = lane % kGatRows;  // entry slot, coil row
-  const int* plan = ws;
-  // the chunks of one work item are consecutive, so they meet in one XCD's L2
-  const int64_t item = xcd_work_item(blockIdx.x);
-  const int64_t wi = item / nchunk;
-  const int chunk = (int)(item - wi * nchunk);
-  if (wi >= plan[tg.o_nwork]) return;
-  const int* wstart = plan + tg.o_wstart;
-  const int t = plan[tg.o_work + wi];
-  const int split = (int)(wi - wstart[t]);
-  const int V = plan[tg.o_start + t + 1] - plan[tg.o_start + t];  // (counts are cleared each call)
-  const int i0 = split * tg.target;
-  const int i1 = min(i0 + tg.target, V);
-  if (i0 >= i1) return;  // no sample is based in this tile
-  const int64_t nrows = g.B * g.C;
-  const int64_t rbase = (int64_t)chunk * kGatRows;
-  const int ncc = (int)((rbase + kGatRows < nrows ? rbase + kGatRows : nrows) - rbase);
-  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kGatT0;
-  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kGatT1;
-
-  // stage region[cell][r] = grid[rbase + r][(o + l) mod K], cell = l0 * kGatR1 + l1:
-  // consecutive threads take consecutive cells of one row (runs of 21 cells are
-  // contiguous in memory); every load is issued before any LDS store.
-  constexpr int kTotal = kGatCells * kGatRows;
-  constexpr int kLoads = (kTotal + kGatThreads - 1) / kGatThreads;  // 18
-  float2 v[kLoads];
-#pragma unroll
-  for (int q = 0; q < kLoads; ++q) {
-    const int idx = threadIdx.x + q * kGatThreads;
-    const int r = idx / kGatCells, cell = idx - r * kGatCells;
-    v[q] = make_float2(0.f, 0.f);
-    if (idx < kTotal && r < ncc) {
-      const int l0 = cell / kGatR1, l1 = cell - l0 * kGatR1;
-      int k0 = o0 + l0, k1 = o1 + l1;
-      while (k0 >= g.K[0]) k0 -= g.K[0];
-      while (k1 >= g.K[1]) k1 -= g.K[1];
-      v[q] = grid[(rbase + r) * g.G + (int64_t)k0 * g.stride[0] + k1];
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < kLoads; ++q) {
-    const int idx = threadIdx.x + q * kGatThreads;
-    const int r = idx / kGatCells, cell = idx - r * kGatCells;
-    if (idx < kTotal) region[cell * kGatPitch + r] = v[q];
-  }
-  __syncthreads();
-
-  // Records are read with vector loads, one entry ahead of use (scalar loads would
-  // share the LDS wait counter), alternating between two register sets.  Half-wave h
-  // of wave w takes entries i0 + 2w + h, stepping by 16.
-  const float4* recs = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) +
-                       (int64_t)plan[tg.o_start + t] * 7;
-  const int J0 = g.J[0], J1 = g.J[1];
-  constexpr int kStep = kGatPerWave * (kGatThreads / 64);
-  float4 ra[7], rb[7];
-  // unconditional loads (the index clamped to the last entry; i0 < i1 here), so the
-  // wait for one set does not also wait for the other set's loads
-  auto load_rec = [&](float4 (&r)[7], int e) {
-    const int ec = e < i1 ? e : i1 - 1;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) r[q] = recs[(int64_t)ec * 7 + q];
-  };
-  auto finish = [&](const float4 (&r)[7], int e) {
-    if (e < i1) {
-      const float2 acc = gat_entry<JF>(r, region, row, J0, J1);
-      if (row < ncc) kdata[(rbase + row) * g.M + __float_as_int(r[6].z)] = acc;
-    }
-  };
-  load_rec(ra, i0 + kGatPerWave * wave + half);
-  for (int eb = i0 + kGatPerWave * wave; eb < i1; eb += 2 * kStep) {
-    const int e = eb + half;
-    load_rec(rb, e + kStep);
-    finish(ra, e);
-    load_rec(ra, e + 2 * kStep);
-    finish(rb, e + kStep);
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1303,6 +759,714 @@ __global__ void __launch_bounds__(256)
       }
       kdata[(rowbase + r0 + c) * g.M + m] = cmul(acc, phase);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// separable multi-coil kernels (2-D, complex64, J <= 6)
+//   The reference's table for dimension d holds t[i] = KB(u) exp(-i pi (N-1) u / K)
+//   at u = (i - J L / 2) / L (_nufft/utils.py:122-179), and the taps j of one sample
+//   read t[i_0 - j L] (the float32 index arithmetic of _nufft/interp.py:129-131 gives
+//   i_j = i_0 - j L except in rare rounding cases).  So along every dimension a tap
+//   weight is R[i_j] (the real KB value) times a factor B shared by the sample's taps
+//   in one tile, times P(k) = exp(i pi (N-1) k / K) of the tap's cell k.  The kernels
+//   work on the grid multiplied by P_0 P_1 (the forward stages its tile rotated; the
+//   adjoint rotates its tile accumulator back when it flushes), with real tap weights:
+//   two FMAs per tap and coil row instead of four.  Entries whose taps break the form
+//   (a table of another form, N unknown, a non-finite sample, or a rounding case) are
+//   flagged by the plan and applied with complex weights in the rotated frame, so the
+//   result is the reference's for every table.
+// ---------------------------------------------------------------------------
+struct SepGeom {
+  int L[2];            // table oversampling
+  int64_t o_rtab[2];   // int offsets of the real tables in the plan
+  int64_t o_nm1;       // int offset of N_d - 1 (2 ints), found from the tables by the plan
+};
+
+// N - 1 of the table's phase, read off its first entry past the centre:
+// t[c + 1] = KB(1 / L) exp(-i pi (N - 1) / (K L)), so N - 1 = -arg(t[c + 1]) K L / pi
+// (the float32 argument is good to far better than one unit of N - 1); the plan then
+// checks every entry against the form, so a wrong estimate only costs speed.
+__device__ __forceinline__ int sep_nm1_estimate(const Geom<float>& g, int d) {
+  const float2 t = g.table[d][g.center[d] + 1];
+  const double ang = (double)atan2f(t.y, t.x);
+  return (int)llrint(-ang * (double)g.K[d] * g.Lf[d] / 3.141592653589793);
+}
+
+// exp(i pi a / q) for an integer a of any sign and q > 0; a is reduced modulo 2q first,
+// so the float angle lies in [0, 2 pi)
+__device__ __forceinline__ float2 pi_phase(int64_t a, int64_t q) {
+  int64_t r = a % (2 * q);
+  if (r < 0) r += 2 * q;
+  // r and q are exact floats below 2^24, and the float division is correctly rounded
+  const float x = q < (1 << 23) ? (float)r / (float)q : (float)((double)r / (double)q);
+  float s, c;
+  sincospif(x, &s, &c);
+  return make_float2(c, s);
+}
+
+// Plan record of one (sample, tile) entry (64 bytes).  Forward: the sample's base cell
+// relative to the tile; adjoint: the footprint's first cell relative to the tile.
+struct SepRec {
+  int me;      // sample index | exception flag << 31
+  int rc;      // (p0 + 8) | (p1 + 8) << 8
+  float2 F;    // forward: shift phase * B_0 * B_1 (exceptions: the shift phase alone);
+               // adjoint: the conjugate
+  float w0[6]; // R_0 of taps 0..5 along dimension 0
+  float w1[6]; // R_1 along dimension 1
+};
+static_assert(sizeof(SepRec) == 64, "plan record is 64 bytes");
+
+
+// Exact reference arithmetic of one sample along dimension d: the unwrapped base cell,
+// the table index of tap 0, and whether every tap j < J uses index i_0 - j L.
+struct SepTaps {
+  int64_t base;
+  int ti0;
+  bool regular;
+};
+__device__ __forceinline__ SepTaps sep_taps(const Geom<float>& g, int d, float omega) {
+#pragma clang fp contract(off)
+  SepTaps r;
+  const float tm = omega / g.gam[d];
+  const bool ok = fabsf(tm) < 1.0e15f;
+  const float fb = floorf(tm - g.halfJ[d]);
+  r.base = ok ? (int64_t)fb + 1 : 0;
+  r.regular = ok;
+  r.ti0 = 0;
+  const int L = (int)g.Lf[d];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (j < g.J[d]) {
+      const float u = tm - (float)(r.base + j);
+      int ti = (int)rintf(u * g.Lf[d]) + g.center[d];
+      ti = ti < 0 ? 0 : (ti > g.tmax[d] ? g.tmax[d] : ti);
+      if (j == 0) r.ti0 = ti;
+      else r.regular = r.regular && ti == r.ti0 - j * L;
+    }
+  }
+  return r;
+}
+
+// Table weight of tap j along dimension d exactly as the reference indexes it (NaN
+// for a non-finite sample), for the complex-weight entries.
+__device__ __forceinline__ float2 sep_tap_weight(const Geom<float>& g, int d, float omega, int j) {
+#pragma clang fp contract(off)
+  const float tm = omega / g.gam[d];
+  if (!(fabsf(tm) < 1.0e15f)) return make_float2(NAN, NAN);
+  if (j >= g.J[d]) return make_float2(0.f, 0.f);
+  const int64_t base = (int64_t)floorf(tm - g.halfJ[d]) + 1;
+  const float u = tm - (float)(base + j);
+  int ti = (int)rintf(u * g.Lf[d]) + g.center[d];
+  ti = ti < 0 ? 0 : (ti > g.tmax[d] ? g.tmax[d] : ti);
+  return g.table[d][ti];
+}
+
+// Complex weights of a forward exception entry in the rotated frame (same index as its
+// record): t_d[i_j] conj(P_d(p_d + j)).
+struct SepX {
+  float2 w0[6];
+  float2 w1[6];
+};
+static_assert(sizeof(SepX) == 96, "exception record is 96 bytes");
+
+__device__ __forceinline__ void sep_exception_weights(const Geom<float>& g, const int (&nm1)[2], float om0, float om1,
+                                                      int64_t p0, int64_t p1, SepX& x) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    x.w0[j] = cmul_conj(sep_tap_weight(g, 0, om0, j), pi_phase((int64_t)nm1[0] * (p0 + j), g.K[0]));
+    x.w1[j] = cmul_conj(sep_tap_weight(g, 1, om1, j), pi_phase((int64_t)nm1[1] * (p1 + j), g.K[1]));
+  }
+}
+
+// B_d = exp(-i pi (N-1) n / (L K)), n = (i_0 - J L / 2) + L p (p: tap 0's cell in the
+// tile frame)
+__device__ __forceinline__ float2 sep_b(const Geom<float>& g, const SepGeom& sg, int nm1, int d, int ti0, int64_t p) {
+  const int64_t L = sg.L[d];
+  const int64_t n = (int64_t)(ti0 - g.center[d]) + L * p;
+  const float2 e = pi_phase((int64_t)nm1 * n, L * g.K[d]);
+  return make_float2(e.x, -e.y);
+}
+
+// Real tables R_d[i] = Re(t_d[i] exp(i pi (N-1)(i - J L / 2) / (L K))) into the plan;
+// flags the plan when any imaginary part is not negligible (the table does not have
+// the form, or N is unknown for a complex table).  Grid-stride over both tables.
+__device__ __forceinline__ void sep_table_prep(const Geom<float>& g, const SepGeom& sg, int* plan, int* flag) {
+  const int nm1[2] = {sep_nm1_estimate(g, 0), sep_nm1_estimate(g, 1)};
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    plan[sg.o_nm1] = nm1[0];
+    plan[sg.o_nm1 + 1] = nm1[1];
+  }
+  const int64_t n0 = g.tmax[0] + 1, n1 = g.tmax[1] + 1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n0 + n1;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int d = i < n0 ? 0 : 1;
+    const int m = (int)(d == 0 ? i : i - n0);
+    const float2 t = g.table[d][m];
+    const float2 p = pi_phase((int64_t)nm1[d] * (m - g.center[d]), (int64_t)sg.L[d] * g.K[d]);
+    const float re = t.x * p.x - t.y * p.y;
+    const float im = t.x * p.y + t.y * p.x;
+    reinterpret_cast<float*>(plan + sg.o_rtab[d])[m] = re;
+    if (!(fabsf(im) <= 0x1p-19f * (fabsf(t.x) + fabsf(t.y)))) atomicOr(flag, 1);
+  }
+}
+
+// ---- plans -------------------------------------------------------------------------
+// Forward: each sample once, in the 16 x 16 tile of its base cell (trajectory order
+// within a tile as far as the building atomics keep it); count[NB] is the table flag.
+__global__ void __launch_bounds__(256)
+    sep_gat_count_kernel(const Geom<float> g, const TileGeom tg, const SepGeom sg, const float* __restrict__ omega,
+                         int* __restrict__ ws, const CacheHdr* cache) {
+  if (cache_hit(cache)) return;  // cached plan reused
+  sep_table_prep(g, sg, ws, ws + tg.o_count + tg.NB);
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = m < g.M;
+  int l[2];
+  const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
+  wave_aggregated_add(ws + tg.o_count, key, live);
+}
+
+__global__ void __launch_bounds__(256)
+    sep_gat_scatter_kernel(const Geom<float> g, const TileGeom tg, const SepGeom sg, const float* __restrict__ omega,
+                           int* __restrict__ ws, const CacheHdr* cache) {
+  if (cache_hit(cache)) return;  // cached plan reused
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = m < g.M;
+  int l[2] = {0, 0};
+  const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
+  const int pos = wave_aggregated_inc(ws + tg.o_cursor, key, live);
+  if (!live) return;
+  const bool tab_ok = ws[tg.o_count + tg.NB] == 0;
+  const int nm10 = ws[sg.o_nm1], nm11 = ws[sg.o_nm1 + 1];
+  const float om0 = omega[m], om1 = omega[g.M + m];
+  const SepTaps a = sep_taps(g, 0, om0), b = sep_taps(g, 1, om1);
+  const float theta = sample_theta<float, 2>(g, omega, m);
+  float2 F = make_float2(cosf(theta), sinf(theta));
+  SepRec rec;
+  const bool exc = !(tab_ok && a.regular && b.regular);
+  const float* R0 = reinterpret_cast<const float*>(ws + sg.o_rtab[0]);
+  const float* R1 = reinterpret_cast<const float*>(ws + sg.o_rtab[1]);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    rec.w0[j] = (!exc && j < g.J[0]) ? R0[a.ti0 - j * sg.L[0]] : 0.f;
+    rec.w1[j] = (!exc && j < g.J[1]) ? R1[b.ti0 - j * sg.L[1]] : 0.f;
+  }
+  if (!exc) {
+    // tap 0's cell in the tile frame is the wrapped base (the staged region continues
+    // past the grid's end without wrapping its coordinates)
+    const int o0 = (key / tg.bstride[0]) % tg.nb[0] * kGatT0, o1 = (key / tg.bstride[1]) % tg.nb[1] * kGatT1;
+    F = cmul(F, cmul(sep_b(g, sg, nm10, 0, a.ti0, o0 + l[0]), sep_b(g, sg, nm11, 1, b.ti0, o1 + l[1])));
+  }
+  rec.me = m | (exc ? (int)0x80000000u : 0);
+  rec.rc = (l[0] + 8) | ((l[1] + 8) << 8);
+  rec.F = F;
+  reinterpret_cast<SepRec*>(reinterpret_cast<char*>(ws) + tg.o_rec_bytes)[pos] = rec;
+  if (exc) {
+    const int o0 = (key / tg.bstride[0]) % tg.nb[0] * kGatT0, o1 = (key / tg.bstride[1]) % tg.nb[1] * kGatT1;
+    const int nm1[2] = {nm10, nm11};
+    SepX x;
+    sep_exception_weights(g, nm1, om0, om1, o0 + l[0], o1 + l[1], x);
+    reinterpret_cast<SepX*>(reinterpret_cast<char*>(ws) + tg.o_rec_bytes + tg.maxent * (int64_t)sizeof(SepRec))[pos] = x;
+  }
+}
+
+// Adjoint: one entry per (sample, 16 x 16 tile its footprint meets); count[NB] is the
+// table flag.  The sample index (with the exception flag) goes to perm, the record to
+// the same position.
+__global__ void __launch_bounds__(256)
+    sep_spr_count_kernel(const Geom<float> g, const TileGeom tg, const SepGeom sg, const float* __restrict__ omega,
+                         int* __restrict__ ws, const CacheHdr* cache) {
+  if (cache_hit(cache)) return;  // cached plan reused
+  sep_table_prep(g, sg, ws, ws + tg.o_count + tg.NB);
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = m < g.M;
+  int tl[2][3], off[2][3], n[2];
+  coil_sample_tiles(g, tg, omega, m, live, tl, off, n);
+  int* count = ws + tg.o_count;
+  coil_for_each_entry(tg, tl, off, n, live, [&](int key, int, int c, bool valid) {
+    const int tkey = valid ? (key - (off[1][c] - kRegMinC0)) / kRegKeys : 0;
+    wave_aggregated_add(count, tkey, valid);
+  });
+}
+
+__global__ void __launch_bounds__(256)
+    sep_spr_scatter_kernel(const Geom<float> g, const TileGeom tg, const SepGeom sg, const float* __restrict__ omega,
+                           int* __restrict__ ws, const CacheHdr* cache) {
+  if (cache_hit(cache)) return;  // cached plan reused
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = m < g.M;
+  int tl[2][3], off[2][3], n[2];
+  coil_sample_tiles(g, tg, omega, m, live, tl, off, n);
+  SepTaps a{}, b{};
+  float2 ns = make_float2(0.f, 0.f);
+  bool regular = false;
+  if (live) {
+    a = sep_taps(g, 0, omega[m]);
+    b = sep_taps(g, 1, omega[g.M + m]);
+    const float theta = sample_theta<float, 2>(g, omega, m);
+    ns = make_float2(cosf(theta), sinf(theta));
+    regular = ws[tg.o_count + tg.NB] == 0 && a.regular && b.regular;
+  }
+  const float* R0 = reinterpret_cast<const float*>(ws + sg.o_rtab[0]);
+  const float* R1 = reinterpret_cast<const float*>(ws + sg.o_rtab[1]);
+  const int nm10 = ws[sg.o_nm1], nm11 = ws[sg.o_nm1 + 1];
+  int* plan = ws;
+  SepRec* recs = reinterpret_cast<SepRec*>(reinterpret_cast<char*>(ws) + tg.o_rec_bytes);
+  coil_for_each_entry(tg, tl, off, n, live, [&](int key, int ai, int ci, bool valid) {
+    const int tkey = valid ? (key - (off[1][ci] - kRegMinC0)) / kRegKeys : 0;
+    const int pos = wave_aggregated_inc(plan + tg.o_cursor, tkey, valid);
+    if (valid) {
+      SepRec rec;
+      float2 F = ns;
+      if (regular) {
+        const int o0 = (tkey / tg.bstride[0]) % tg.nb[0] * kRegT0, o1 = (tkey / tg.bstride[1]) % tg.nb[1] * kRegT1;
+        F = cmul(F, cmul(sep_b(g, sg, nm10, 0, a.ti0, (int64_t)o0 + off[0][ai]),
+                         sep_b(g, sg, nm11, 1, b.ti0, (int64_t)o1 + off[1][ci])));
+      }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        rec.w0[j] = (regular && j < g.J[0]) ? R0[a.ti0 - j * sg.L[0]] : 0.f;
+        rec.w1[j] = (regular && j < g.J[1]) ? R1[b.ti0 - j * sg.L[1]] : 0.f;
+      }
+      rec.me = m | (regular ? 0 : (int)0x80000000u);
+      rec.rc = (off[0][ai] + 8) | ((off[1][ci] + 8) << 8);
+      rec.F = make_float2(F.x, -F.y);
+      plan[tg.o_perm + pos] = rec.me;
+      recs[pos] = rec;
+    }
+  });
+}
+
+// Scan of per-tile counts into tile starts and the work list: tiles above `maxunsplit`
+// entries are cut into pieces of `target`, listed first (longest work first); empty
+// tiles get no item when `skip_empty`.
+__global__ void __launch_bounds__(1024) sep_plan_scan_kernel(const TileGeom tg, int* __restrict__ ws,
+                                                            const CacheHdr* cache, int maxunsplit, int target,
+                                                            int skip_empty) {
+  if (cache_hit(cache)) return;  // cached plan reused
+  __shared__ int sbuf[1024];
+  int* plan = ws;
+  const int* count = plan + tg.o_count;
+  int* start = plan + tg.o_start;
+  int* cursor = plan + tg.o_cursor;
+  int* wstart = plan + tg.o_wstart;
+  int* work = plan + tg.o_work;
+  int* hot = plan + tg.o_hot;
+  int* cold = plan + tg.o_cold;
+  const int NB = tg.NB;
+  block_exclusive_scan(count, start, NB, sbuf);
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    cursor[t] = start[t];
+    const int v = count[t];
+    const int ns = v <= maxunsplit ? 1 : (v + target - 1) / target;
+    work[t] = ns > 1 ? ns : 0;
+    wstart[t] = (ns == 1 && (v > 0 || !skip_empty)) ? 1 : 0;
+  }
+  __syncthreads();
+  block_exclusive_scan(work, hot, NB, sbuf);
+  block_exclusive_scan(wstart, cold, NB, sbuf);
+  const int nhot = hot[NB];
+  const int cap = (int)tg.maxwork;
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int ns = hot[t + 1] - hot[t];
+    wstart[t] = ns > 0 ? hot[t] : nhot + cold[t];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int ns = hot[t + 1] - hot[t];
+    const int items = ns > 0 ? ns : cold[t + 1] - cold[t];
+    const int a = wstart[t], e = min(a + items, cap);
+    for (int w = a; w < e; ++w) work[w] = t;
+  }
+  if (threadIdx.x == 0) plan[tg.o_nwork] = min(nhot + cold[NB], cap);
+}
+
+// ---- forward gather ------------------------------------------------------------------
+// A group per 16 x 16 tile, piece and chunk of 16 coil rows; the tile and its halo are
+// staged in LDS with the coil row fastest, rotated: region = grid * P_0 * P_1; a
+// quarter-wave per entry computes
+//   y = F * sum_i w0[i] sum_j w1[j] region[l0 + i][l1 + j]
+// with real w0, w1; exception entries use complex weights t_d * conj(P_d).
+__device__ __forceinline__ float2 sep_gat_real(const float (&w0)[6], const float (&w1)[6], const float2* L0, int J0,
+                                               int J1, bool fixed6) {
+  float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int j0 = 0; j0 < 6; ++j0) {
+    if (fixed6 || j0 < J0) {
+      const float2* L = L0 + j0 * (kGatR1 * kGatPitch);
+      float2 p = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j1 = 0; j1 < 6; ++j1) {
+        if (fixed6 || j1 < J1) {
+          const float2 x = L[j1 * kGatPitch];
+          p.x = fmaf(w1[j1], x.x, p.x);
+          p.y = fmaf(w1[j1], x.y, p.y);
+        }
+      }
+      acc.x = fmaf(w0[j0], p.x, acc.x);
+      acc.y = fmaf(w0[j0], p.y, acc.y);
+    }
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float2 sep_gat_complex(const SepX* x, const float2* L0, int J0, int J1) {
+  float2 acc = make_float2(0.f, 0.f);
+  for (int j0 = 0; j0 < J0; ++j0) {
+    const float2* L = L0 + j0 * (kGatR1 * kGatPitch);
+    float2 p = make_float2(0.f, 0.f);
+    for (int j1 = 0; j1 < J1; ++j1) cmac(p, x->w1[j1], L[j1 * kGatPitch]);
+    cmac(acc, x->w0[j0], p);
+  }
+  return acc;
+}
+
+template <int JF>
+__global__ void __launch_bounds__(kGatThreads)
+    gather_sep_kernel(const Geom<float> g, const TileGeom tg, const SepGeom sg, const float2* __restrict__ grid,
+                      const int* ws, float2* kdata, int nchunk) {
+  __shared__ __attribute__((aligned(16))) float2 region[kGatCells * kGatPitch];
+  __shared__ float2 cph[kGatCells];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int half = lane / kGatRows, row = lane % kGatRows;  // entry slot, coil row
+  const int* plan = ws;
+  const int64_t item = xcd_work_item(blockIdx.x);
+  const int64_t wi = item / nchunk;
+  const int chunk = (int)(item - wi * nchunk);
+  if (wi >= plan[tg.o_nwork]) return;
+  const int t = plan[tg.o_work + wi];
+  const int split = (int)(wi - plan[tg.o_wstart + t]);
+  const int V = plan[tg.o_start + t + 1] - plan[tg.o_start + t];
+  const int i0 = split * tg.target;
+  const int i1 = min(i0 + tg.target, V);
+  if (i0 >= i1) return;
+  const int64_t nrows = g.B * g.C;
+  const int64_t rbase = (int64_t)chunk * kGatRows;
+  const int ncc = (int)((rbase + kGatRows < nrows ? rbase + kGatRows : nrows) - rbase);
+  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kGatT0;
+  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kGatT1;
+  const int nm1[2] = {plan[sg.o_nm1], plan[sg.o_nm1 + 1]};
+
+  // Stage: thread (coil row sr, region column sl1) of the first 16 x 21 threads loads
+  // that column's 21 region rows (consecutive threads read consecutive grid columns;
+  // the row index steps by one and wraps), while the other threads compute the cell
+  // phases; then every loaded value is stored rotated.
+  constexpr int kStageThreads = kGatRows * kGatR1;  // 336
+  static_assert(kStageThreads < kGatThreads, "spare threads compute the phases");
+  const int sr = (int)threadIdx.x / kGatR1, sl1 = (int)threadIdx.x - sr * kGatR1;
+  float2 v[kGatR0];
+  if ((int)threadIdx.x < kStageThreads) {
+    int k1 = o1 + sl1;
+    while (k1 >= g.K[1]) k1 -= g.K[1];
+    int k0 = o0;  // a tile origin lies inside the grid
+    const bool live = sr < ncc;
+    const float2* src = grid + (rbase + (live ? sr : 0)) * g.G + k1;
+#pragma unroll
+    for (int l0 = 0; l0 < kGatR0; ++l0) {
+      v[l0] = live ? src[(int64_t)k0 * g.stride[0]] : make_float2(0.f, 0.f);
+      k0 = k0 + 1 == g.K[0] ? 0 : k0 + 1;
+    }
+  } else {
+    for (int c = (int)threadIdx.x - kStageThreads; c < kGatCells; c += kGatThreads - kStageThreads) {
+      const int l0 = c / kGatR1, l1 = c - l0 * kGatR1;
+      cph[c] = cmul(pi_phase((int64_t)nm1[0] * (o0 + l0), g.K[0]), pi_phase((int64_t)nm1[1] * (o1 + l1), g.K[1]));
+    }
+  }
+  lds_barrier();
+  if ((int)threadIdx.x < kStageThreads) {
+#pragma unroll
+    for (int l0 = 0; l0 < kGatR0; ++l0) {
+      const int cell = l0 * kGatR1 + sl1;
+      region[cell * kGatPitch + sr] = cmul(v[l0], cph[cell]);
+    }
+  }
+  __syncthreads();
+
+  // Records one entry ahead, alternating between two register sets (unconditional,
+  // clamped loads).  Half-wave h of wave w takes entries i0 + 4w + h, stepping by 32.
+  const float4* recs = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) +
+                       (int64_t)plan[tg.o_start + t] * 4;
+  const SepX* xrecs = reinterpret_cast<const SepX*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes +
+                                                    tg.maxent * (int64_t)sizeof(SepRec));
+  const int J0 = g.J[0], J1 = g.J[1];
+  constexpr int kStep = kGatPerWave * (kGatThreads / 64);
+  float4 ra[4], rb[4];
+  auto load_rec = [&](float4 (&r)[4], int e) {
+    const int ec = e < i1 ? e : i1 - 1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r[q] = recs[(int64_t)ec * 4 + q];
+  };
+  auto finish = [&](const float4 (&r)[4], int e) {
+    if (e < i1) {
+      const int me = __float_as_int(r[0].x), rc = __float_as_int(r[0].y);
+      const int l0 = (rc & 255) - 8, l1 = ((rc >> 8) & 255) - 8;
+      const float2 F = make_float2(r[0].z, r[0].w);
+      const float2* L0 = region + (l0 * kGatR1 + l1) * kGatPitch + row;
+      float2 acc;
+      if (me >= 0) {
+        const float w0[6] = {r[1].x, r[1].y, r[1].z, r[1].w, r[2].x, r[2].y};
+        const float w1[6] = {r[2].z, r[2].w, r[3].x, r[3].y, r[3].z, r[3].w};
+        acc = sep_gat_real(w0, w1, L0, J0, J1, JF == 6);
+      } else {
+        acc = sep_gat_complex(xrecs + (e + (int64_t)plan[tg.o_start + t]), L0, J0, J1);
+      }
+      if (row < ncc) kdata[(rbase + row) * g.M + (me & 0x7fffffff)] = cmul(acc, F);
+    }
+  };
+  load_rec(ra, i0 + kGatPerWave * wave + half);
+  for (int eb = i0 + kGatPerWave * wave; eb < i1; eb += 2 * kStep) {
+    const int e = eb + half;
+    load_rec(rb, e + kStep);
+    finish(ra, e);
+    load_rec(ra, e + 2 * kStep);
+    finish(rb, e + kStep);
+  }
+}
+
+// ---- adjoint spread --------------------------------------------------------------------
+// A group per 16 x 16 tile, split and chunk of 32 coil rows; half-wave h of wave w owns
+// tile row 2w + h of its coil row in registers.  In the rotated frame: the staged data
+// are s = kdata * F, a tap adds s * w0[i] * w1[j] with real weights, and the flush
+// multiplies each cell by conj(P_0 P_1).  Exception entries have zero real weights
+// here; after the flush the group adds them with their complex weights conj(t_d)
+// (looked up from omega) and atomics.
+struct SepLds {
+  float w0r[6];  // R_0 of taps 0..5 (zero for exception entries)
+  float w1r[6];  // R_1
+  int r0, c0, pad[2];
+};
+static_assert(sizeof(SepLds) == 64, "staged record is 64 bytes");
+
+template <int C0>
+__device__ __forceinline__ void sep_add_row_real(float2 (&acc)[kRegT1], const float2 u, const float (&w1)[6]) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (C0 + j >= 0 && C0 + j < kRegT1) {
+      acc[C0 + j].x = fmaf(u.x, w1[j], acc[C0 + j].x);
+      acc[C0 + j].y = fmaf(u.y, w1[j], acc[C0 + j].y);
+    }
+  }
+}
+
+template <int C0>
+__device__ __forceinline__ void sep_entry(float2 (&acc)[kRegT1], const SepLds* rec, const float2 s, int j0, int J0) {
+  float w1[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) w1[j] = rec->w1r[j];
+  const float w0 = rec->w0r[min(max(j0, 0), 5)];
+  if (j0 >= 0 && j0 < J0) sep_add_row_real<C0>(acc, make_float2(s.x * w0, s.y * w0), w1);
+}
+
+__global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_eu(6)))
+    spread_sep_kernel(const Geom<float> g, const TileGeom tg, const SepGeom sg, const float2* __restrict__ kdata,
+                      const float* __restrict__ omega, const int* __restrict__ ws, float2* __restrict__ grid,
+                      int nchunk) {
+  __shared__ __attribute__((aligned(16))) float2 vals[kRegBatch * kRegPitch];
+  __shared__ __attribute__((aligned(16))) SepLds brec[kRegBatch];
+  __shared__ float2 rph[kRegT0], cphs[kRegT1];  // conj(P_0) of the tile rows, conj(P_1) of its columns
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int coil = lane & (kRegCoils - 1), hh = lane / kRegCoils;
+
+  const int* plan = ws;
+  const int64_t wi = blockIdx.x / nchunk;
+  const int chunk = (int)(blockIdx.x - wi * nchunk);
+  if (wi >= plan[tg.o_nwork]) return;
+  const int t = plan[tg.o_work + wi];
+  const int split = (int)(wi - plan[tg.o_wstart + t]);
+  const int64_t nrows = g.B * g.C;
+  const int64_t rbase = (int64_t)chunk * kRegCoils;
+  const int ncc = (int)((rbase + kRegCoils < nrows ? rbase + kRegCoils : nrows) - rbase);
+  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kRegT0;
+  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kRegT1;
+  const int cstart = plan[tg.o_start + t];
+  const int V = plan[tg.o_start + t + 1] - cstart;
+  const int nsplit = reg_splits(V);
+  const int i0 = nsplit > 1 ? split * kRegHotTarget : 0;
+  const int i1 = nsplit > 1 ? min(i0 + kRegHotTarget, V) : V;
+  const int* perm = plan + tg.o_perm + cstart;
+  const SepRec* recs = reinterpret_cast<const SepRec*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) + cstart;
+  const int J0 = g.J[0];
+  const int wrow = wave * kRegRowsPerWave;
+  const int nm1[2] = {plan[sg.o_nm1], plan[sg.o_nm1 + 1]};
+
+  if (threadIdx.x < kRegT0) {
+    const float2 p = pi_phase((int64_t)nm1[0] * (o0 + (int)threadIdx.x), g.K[0]);
+    rph[threadIdx.x] = make_float2(p.x, -p.y);
+  } else if (threadIdx.x < kRegT0 + kRegT1) {
+    const float2 p = pi_phase((int64_t)nm1[1] * (o1 + (int)threadIdx.x - kRegT0), g.K[1]);
+    cphs[threadIdx.x - kRegT0] = make_float2(p.x, -p.y);
+  }
+  lds_barrier();  // the flush reads them even when the tile has no entries
+
+  float2 acc[kRegT1];
+#pragma unroll
+  for (int c = 0; c < kRegT1; ++c) acc[c] = make_float2(0.f, 0.f);
+
+  // Software pipeline: the next batch's data (with its entry's factor F) and records
+  // are loaded into registers while a batch is processed, and the sample indices one
+  // batch further.  Thread (wave w, lane e) stages coil rows
+  // 4w .. 4w+3 of entry e; threads of waves 0..3 load one float4 of the 64-byte records.
+  constexpr int kPer = kRegCoils / kRegWaves;
+  float4 prec = make_float4(0.f, 0.f, 0.f, 0.f);
+  float2 pv[kPer];
+  float2 pF = make_float2(0.f, 0.f);
+  int pme = 0;
+  bool anyexc = false;  // an exception entry was staged (the same in every wave)
+  auto prefetch = [&](int b, int me) {
+    const int n = min(kRegBatch, i1 - b);
+    if ((int)threadIdx.x < n * 4) prec = reinterpret_cast<const float4*>(recs + b)[threadIdx.x];
+    pme = lane < n ? me : 0;
+    pF = lane < n ? recs[b + lane].F : make_float2(0.f, 0.f);
+    const int mb = me & 0x7fffffff;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int row = wave * kPer + q;
+      pv[q] = make_float2(0.f, 0.f);
+      if (lane < n && row < ncc) pv[q] = kdata[(rbase + row) * g.M + mb];
+    }
+  };
+  int mnext = 0;
+  if (i0 < i1) {
+    const int m0 = lane < min(kRegBatch, i1 - i0) ? perm[i0 + lane] : 0;
+    prefetch(i0, m0);
+    const int b1 = i0 + kRegBatch;
+    if (b1 < i1 && lane < min(kRegBatch, i1 - b1)) mnext = perm[b1 + lane];
+  }
+  for (int base = i0; base < i1; base += kRegBatch) {
+    const int nb = min(kRegBatch, i1 - base);
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) vals[lane * kRegPitch + wave * kPer + q] = cmul(pv[q], pF);
+    anyexc = anyexc || __ballot(pme < 0) != 0;
+    // records: float4 q of entry e holds (me, rc, F) for q = 0, then w0[0..3],
+    // w0[4..5] w1[0..1], w1[2..5]: the latter three are the first 48 bytes of SepLds
+    if ((int)threadIdx.x < nb * 4) {
+      const int e = threadIdx.x >> 2, q = threadIdx.x & 3;
+      if (q == 0) {
+        const int rc = __float_as_int(prec.y);
+        brec[e].r0 = (rc & 255) - 8;
+        brec[e].c0 = ((rc >> 8) & 255) - 8;
+      } else {
+        reinterpret_cast<float4*>(brec + e)[q - 1] = prec;
+      }
+    }
+    __syncthreads();
+    const int r0l = lane < nb ? brec[lane].r0 : 1000, c0l = lane < nb ? brec[lane].c0 : 1000;
+    const int b1 = base + kRegBatch;
+    if (b1 < i1) {
+      prefetch(b1, mnext);
+      const int b2 = b1 + kRegBatch;
+      mnext = (b2 < i1 && lane < min(kRegBatch, i1 - b2)) ? perm[b2 + lane] : 0;
+    }
+    const int jal = wrow - r0l;
+    const unsigned long long hits = __ballot(jal + kRegRowsPerWave > 0 && jal < J0);
+#define TKBN_SEP_GROUP(C0)                                                                        \
+  {                                                                                               \
+    unsigned long long mk = __ballot(c0l == (C0)) & hits;                                         \
+    while (mk) {                                                                                  \
+      const int c = __ffsll(mk) - 1;                                                              \
+      mk &= mk - 1;                                                                               \
+      const int ja = wrow - __builtin_amdgcn_readlane(r0l, c);                                    \
+      sep_entry<C0>(acc, brec + c, vals[c * kRegPitch + coil], ja + hh, J0);                      \
+    }                                                                                             \
+  }
+    TKBN_SEP_GROUP(-5) TKBN_SEP_GROUP(-4) TKBN_SEP_GROUP(-3) TKBN_SEP_GROUP(-2) TKBN_SEP_GROUP(-1)
+    TKBN_SEP_GROUP(0) TKBN_SEP_GROUP(1) TKBN_SEP_GROUP(2) TKBN_SEP_GROUP(3) TKBN_SEP_GROUP(4)
+    TKBN_SEP_GROUP(5) TKBN_SEP_GROUP(6) TKBN_SEP_GROUP(7) TKBN_SEP_GROUP(8) TKBN_SEP_GROUP(9)
+    TKBN_SEP_GROUP(10) TKBN_SEP_GROUP(11) TKBN_SEP_GROUP(12) TKBN_SEP_GROUP(13) TKBN_SEP_GROUP(14)
+    TKBN_SEP_GROUP(15)
+#undef TKBN_SEP_GROUP
+    __syncthreads();
+  }
+
+  // Flush: every owned cell inside the grid is written (split tiles were zeroed first
+  // and add), rotated back by conj(P_0 P_1).  The rows go out through LDS in rounds of
+  // kRound tile rows so that each store instruction writes 16 consecutive cells (128 B)
+  // of every coil row; LDS-only barriers keep the stores in flight.
+  constexpr int kRound = 4;
+  constexpr int kRoundWaves = kRound / kRegRowsPerWave;
+  for (int round = 0; round < kRegT0 / kRound; ++round) {
+    if (wave / kRoundWaves == round) {
+      const int slot = (wave % kRoundWaves) * kRegRowsPerWave + hh;
+      const float2 pr = rph[kRound * round + slot];
+#pragma unroll
+      for (int c = 0; c < kRegT1; ++c) vals[(slot * kRegT1 + c) * kRegPitch + coil] = cmul(acc[c], cmul(pr, cphs[c]));
+    }
+    lds_barrier();
+#pragma unroll
+    for (int q = 0; q < (kRound * kRegT1 * kRegCoils) / kRegThreads; ++q) {
+      const int e = threadIdx.x + q * kRegThreads;
+      const int c = e & (kRegT1 - 1), cl = (e / kRegT1) & (kRegCoils - 1), slot = e / (kRegT1 * kRegCoils);
+      const int k0 = o0 + kRound * round + slot, k1 = o1 + c;
+      if (cl < ncc && k0 < g.K[0] && k1 < g.K[1]) {
+        const float2 v = vals[(slot * kRegT1 + c) * kRegPitch + cl];
+        float2* dst = grid + (rbase + cl) * g.G + (int64_t)k0 * g.stride[0] + k1;
+        if (nsplit == 1) {
+          *dst = v;
+        } else {
+          unsafeAtomicAdd(&dst->x, v.x);
+          unsafeAtomicAdd(&dst->y, v.y);
+        }
+      }
+    }
+    lds_barrier();
+  }
+
+  // Exception entries (complex weights; none for the reference's own tables except
+  // rare rounding cases) are added with atomics after the tile's stores are complete:
+  // thread (tap row i, coil row) adds s * w0[i] * w1[j] over the row's in-tile taps.
+  if (!anyexc) return;
+  __syncthreads();  // waits for the flush stores
+  for (int base = i0; base < i1; base += 64) {
+    const int e = base + lane;
+    const unsigned long long exc = __ballot(e < i1 && perm[min(e, i1 - 1)] < 0);
+    unsigned long long mk = exc;
+    while (mk) {
+      const int c = __ffsll(mk) - 1;
+      mk &= mk - 1;
+      const int ei = base + c;
+      const int m = perm[ei] & 0x7fffffff;
+      const int i = threadIdx.x >> 5;  // tap row 0..15 (0..5 used)
+      if (i < J0 && coil < ncc) {
+        const SepRec& r = recs[ei];
+        const int r0 = (r.rc & 255) - 8, c0 = ((r.rc >> 8) & 255) - 8;
+        const int k0 = r0 + i;
+        if (k0 >= 0 && k0 < kRegT0 && o0 + k0 < g.K[0]) {
+          const float om0 = omega[m], om1 = omega[g.M + m];
+          const float2 t0 = sep_tap_weight(g, 0, om0, i);
+          const float2 u = cmul(cmul(kdata[(rbase + coil) * g.M + m], r.F), make_float2(t0.x, -t0.y));
+          for (int j = 0; j < g.J[1]; ++j) {
+            const int k1 = c0 + j;
+            if (k1 >= 0 && k1 < kRegT1 && o1 + k1 < g.K[1]) {
+              const float2 t1 = sep_tap_weight(g, 1, om1, j);
+              const float2 v = cmul(u, make_float2(t1.x, -t1.y));
+              float2* dst = grid + (rbase + coil) * g.G + (int64_t)(o0 + k0) * g.stride[0] + (o1 + k1);
+              unsafeAtomicAdd(&dst->x, v.x);
+              unsafeAtomicAdd(&dst->y, v.y);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// zeroes the tiles whose pieces add into the grid
+__global__ void __launch_bounds__(256)
+    sep_zero_hot_kernel(const Geom<float> g, const TileGeom tg, const int* __restrict__ ws, float2* __restrict__ grid) {
+  const int t = blockIdx.x;
+  const int* start = ws + tg.o_start;
+  if (reg_splits(start[t + 1] - start[t]) <= 1) return;
+  const int64_t nrows = g.B * g.C;
+  const int64_t r0 = (int64_t)blockIdx.y * 64;
+  const int64_t r1 = r0 + 64 < nrows ? r0 + 64 : nrows;
+  const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kRegT0;
+  const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kRegT1;
+  for (int cell = threadIdx.x; cell < kRegT0 * kRegT1; cell += blockDim.x) {
+    const int k0 = o0 + cell / kRegT1, k1 = o1 + cell % kRegT1;
+    if (k0 >= g.K[0] || k1 >= g.K[1]) continue;
+    for (int64_t r = r0; r < r1; ++r) grid[r * g.G + (int64_t)k0 * g.stride[0] + k1] = make_float2(0.f, 0.f);
   }
 }
 
@@ -1545,10 +1709,6 @@ static TileGeom make_coil_tile_geom(const tkbn_interp_desc* d) {
   return tg;
 }
 
-static int64_t coil_workspace_bytes(const TileGeom& tg) {
-  return tg.o_rec_bytes + tg.maxent * (int64_t)sizeof(CoilRec);
-}
-
 // register-accumulation spread: complex64, 2-D, shared trajectory, J <= 6, enough
 // coil rows to fill the lanes, and grids of at least tile + J - 1 cells per dimension
 static bool coil_variant_eligible(const tkbn_interp_desc* d) {
@@ -1557,10 +1717,6 @@ static bool coil_variant_eligible(const tkbn_interp_desc* d) {
   if (d->batch * d->coils < 32) return false;
   return d->grid_size[0] >= kRegT0 + 5 && d->grid_size[1] >= kRegT1 + 5;
 }
-
-// plan kinds (part of the cache key)
-constexpr int kPlanSpreadCoils = 1;
-constexpr int kPlanGatherCoils = 2;
 
 // One-shot timing hook: events the caller asked to have recorded around the next
 // main interpolation kernel (tkbn_profile_next_kernel).
@@ -1577,59 +1733,162 @@ static void profile_after(hipStream_t s) {
   g_profile = ProfileHook{};
 }
 
-// count -> scan -> scatter of the multi-coil spread plan (each exits at once on a cache
-// hit); the check kernel has cleared the counts
-static int coil_spread_plan(const Geom<float>& g, const TileGeom& tg, const void* omega, int* wsi,
-                            const CacheHdr* cache, hipStream_t s) {
-  dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
-  if (g.M > 0) {
-    coil_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-    TKBN_LAUNCH_CHECK("coil_bin_count_kernel");
+static TileGeom make_gather_geom(const tkbn_interp_desc* d);
+
+// ---- separable multi-coil kernels: plans and launches ----------------------------------
+constexpr int kPlanSepGather = 6;
+constexpr int kPlanSepSpread = 7;
+
+// N - 1 and the real tables after the plan's int arrays; records 64-byte aligned after them
+static void sep_finish_layout(TileGeom& tg, SepGeom& sg, int64_t o, const tkbn_interp_desc* d) {
+  sg.o_nm1 = o;
+  o += 2;
+  for (int i = 0; i < 2; ++i) {
+    sg.o_rtab[i] = o;
+    o += d->numpoints[i] * d->table_oversamp[i] + 1;
   }
-  coil_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
-  TKBN_LAUNCH_CHECK("coil_plan_scan_kernel");
-  if (g.M > 0) {
-    coil_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-    TKBN_LAUNCH_CHECK("coil_bin_scatter_kernel");
-  }
-  return TKBN_OK;
+  o = (o + 15) & ~15LL;
+  tg.plan_ints = o;
+  tg.o_rec_bytes = o * 4;
 }
 
-static int spread_coils_launch(const tkbn_interp_desc* d, const void* kdata, const void* omega, void* ws,
-                               void* grid, hipStream_t s) {
+static SepGeom sep_phase_of(const tkbn_interp_desc* d) {
+  SepGeom sg{};
+  for (int i = 0; i < 2; ++i) sg.L[i] = (int)d->table_oversamp[i];
+  return sg;
+}
+
+static TileGeom make_sep_gather_geom(const tkbn_interp_desc* d, SepGeom* sg) {
+  TileGeom tg = make_gather_geom(d);
+  *sg = sep_phase_of(d);
+  int64_t o = 0;
+  tg.o_key = o;
+  tg.o_perm = o;  // unused: the records carry the sample index
+  tg.o_count = o; o += tg.NB + 1;  // + the table flag
+  tg.o_start = o; o += tg.NB + 1;
+  tg.o_cursor = o; o += tg.NB;
+  tg.o_wstart = o; o += tg.NB + 1;
+  tg.o_work = o; o += (tg.maxwork > tg.NB ? tg.maxwork : tg.NB);
+  tg.o_nwork = o; o += 1;
+  tg.o_hot = o; o += tg.NB + 1;
+  tg.o_cold = o; o += tg.NB + 1;
+  sep_finish_layout(tg, *sg, o, d);
+  return tg;
+}
+
+static TileGeom make_sep_spread_geom(const tkbn_interp_desc* d, SepGeom* sg) {
+  TileGeom tg = make_coil_tile_geom(d);
+  *sg = sep_phase_of(d);
+  int64_t o = 0;
+  tg.o_key = o;
+  tg.o_perm = o; o += tg.maxent;
+  tg.o_count = o; o += tg.NB + 1;  // + the table flag
+  tg.o_start = o; o += tg.NB + 1;
+  tg.o_cursor = o; o += tg.NB;
+  tg.o_wstart = o; o += tg.NB + 1;
+  tg.o_work = o; o += (tg.maxwork > tg.NB ? tg.maxwork : tg.NB);
+  tg.o_nwork = o; o += 1;
+  tg.o_hot = o; o += tg.NB + 1;
+  tg.o_cold = o; o += tg.NB + 1;
+  sep_finish_layout(tg, *sg, o, d);
+  return tg;
+}
+
+// records (the gather's followed by its exception records at the same indices)
+static int64_t sep_workspace_bytes(const TileGeom& tg, bool gather) {
+  return tg.o_rec_bytes + tg.maxent * (int64_t)(sizeof(SepRec) + (gather ? sizeof(SepX) : 0));
+}
+
+static int gather_sep_launch(const tkbn_interp_desc* d, const void* grid, const void* omega, void* kdata, void* ws,
+                             hipStream_t s) {
   const Geom<float> g = make_geom<float>(d);
-  const TileGeom tg = make_coil_tile_geom(d);
+  SepGeom sg;
+  const TileGeom tg = make_sep_gather_geom(d, &sg);
   int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
-  int st = TKBN_OK;
+  const float* om = static_cast<const float*>(omega);
   if (!plan_reused(d)) {
     CacheHdr* cache = nullptr;
     CacheZero z;
     z.p = wsi + tg.o_count;
-    z.n = tg.nkeys;
-    st = plan_cache_begin(d, kPlanSpreadCoils, tg.T, omega, ws, s, &cache, z);
+    z.n = tg.NB + 1;
+    int st = plan_cache_begin(d, kPlanSepGather, tg.T, omega, ws, s, &cache, z);
     if (st != TKBN_OK) return st;
-    st = coil_spread_plan(g, tg, omega, wsi, cache, s);
-    if (st != TKBN_OK) {
+    dim3 gc((unsigned)std::max<int64_t>(1, ceil_div(g.M, 256)), 1, 1);
+    sep_gat_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, sg, om, wsi, cache);
+    hipError_t e1 = hipGetLastError();
+    sep_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache, tg.target, tg.target, 1);
+    hipError_t e2 = hipGetLastError();
+    sep_gat_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, sg, om, wsi, cache);
+    hipError_t e3 = hipGetLastError();
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
       plan_cache_invalidate(ws, s);
-      return st;
+      set_error("launch of the separable gather plan failed: %s",
+                hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3)));
+      return TKBN_ERR_HIP;
     }
   } else {
-    st = plan_registry_require(d, kPlanSpreadCoils, tg.T, ws);
+    const int st = plan_registry_require(d, kPlanSepGather, tg.T, ws);
+    if (st != TKBN_OK) return st;
+  }
+  const int nchunk = (int)ceil_div(g.B * g.C, kGatRows);
+  const int64_t groups = ceil_div(tg.maxwork * nchunk, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
+  TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_UNSUPPORTED, "too many coil rows x work items for one launch");
+  profile_before(s);
+  if (g.J[0] == 6 && g.J[1] == 6)
+    gather_sep_kernel<6><<<dim3((unsigned)groups), dim3(kGatThreads), 0, s>>>(
+        g, tg, sg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata), nchunk);
+  else
+    gather_sep_kernel<0><<<dim3((unsigned)groups), dim3(kGatThreads), 0, s>>>(
+        g, tg, sg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata), nchunk);
+  profile_after(s);
+  TKBN_LAUNCH_CHECK("gather_sep_kernel");
+  note_kernel("gather_sep_kernel");
+  return TKBN_OK;
+}
+
+static int spread_sep_launch(const tkbn_interp_desc* d, const void* kdata, const void* omega, void* ws, void* grid,
+                             hipStream_t s) {
+  const Geom<float> g = make_geom<float>(d);
+  SepGeom sg;
+  const TileGeom tg = make_sep_spread_geom(d, &sg);
+  int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
+  const float* om = static_cast<const float*>(omega);
+  if (!plan_reused(d)) {
+    CacheHdr* cache = nullptr;
+    CacheZero z;
+    z.p = wsi + tg.o_count;
+    z.n = tg.NB + 1;
+    int st = plan_cache_begin(d, kPlanSepSpread, tg.T, omega, ws, s, &cache, z);
+    if (st != TKBN_OK) return st;
+    dim3 gc((unsigned)std::max<int64_t>(1, ceil_div(g.M, 256)), 1, 1);
+    sep_spr_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, sg, om, wsi, cache);
+    hipError_t e1 = hipGetLastError();
+    sep_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache, kRegMaxUnsplit, kRegHotTarget, 0);
+    hipError_t e2 = hipGetLastError();
+    sep_spr_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, sg, om, wsi, cache);
+    hipError_t e3 = hipGetLastError();
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+      plan_cache_invalidate(ws, s);
+      set_error("launch of the separable spread plan failed: %s",
+                hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3)));
+      return TKBN_ERR_HIP;
+    }
+  } else {
+    const int st = plan_registry_require(d, kPlanSepSpread, tg.T, ws);
     if (st != TKBN_OK) return st;
   }
   const int64_t nrows = g.B * g.C;
   const unsigned ychunks = (unsigned)ceil_div(nrows, 64);
-  coil_zero_hot_kernel<<<dim3((unsigned)tg.NB, ychunks, 1), dim3(256), 0, s>>>(
-      g, tg, wsi, static_cast<float2*>(grid));
-  TKBN_LAUNCH_CHECK("coil_zero_hot_kernel");
+  sep_zero_hot_kernel<<<dim3((unsigned)tg.NB, ychunks, 1), dim3(256), 0, s>>>(g, tg, wsi, static_cast<float2*>(grid));
+  TKBN_LAUNCH_CHECK("sep_zero_hot_kernel");
   const int nchunk = (int)ceil_div(nrows, kRegCoils);
   TKBN_REQUIRE(tg.maxwork * nchunk < (1LL << 31), TKBN_ERR_UNSUPPORTED, "too many coil rows x work items for one launch");
   profile_before(s);
-  spread_coil_regs_kernel<<<dim3((unsigned)(tg.maxwork * nchunk)), dim3(kRegThreads), 0, s>>>(
-      g, tg, static_cast<const float2*>(kdata), wsi, static_cast<float2*>(grid), nchunk);
+  spread_sep_kernel<<<dim3((unsigned)(tg.maxwork * nchunk)), dim3(kRegThreads), 0, s>>>(
+      g, tg, sg, static_cast<const float2*>(kdata), om, wsi, static_cast<float2*>(grid), nchunk);
   profile_after(s);
-  TKBN_LAUNCH_CHECK("spread_coil_regs_kernel");
-  note_kernel("spread_coil_regs_kernel");
+  TKBN_LAUNCH_CHECK("spread_sep_kernel");
+  note_kernel("spread_sep_kernel");
   return TKBN_OK;
 }
 
@@ -1647,7 +1906,7 @@ template <typename T>
 int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const void* omega, void* grid, void* ws,
                  hipStream_t s) {
   if (std::is_same<T, float>::value && !det && coil_variant_eligible(d))
-    return spread_coils_launch(d, kdata, omega, ws, grid, s);
+    return spread_sep_launch(d, kdata, omega, ws, grid, s);
   const Geom<T> g = make_geom<T>(d);
   const TileGeom tg = make_tile_geom(d, det);
   const size_t lds = spread_lds_bytes(d, tg, det);
@@ -1778,64 +2037,11 @@ static TileGeom make_gather_geom(const tkbn_interp_desc* d) {
   return tg;
 }
 
-static int64_t gather_workspace_bytes(const TileGeom& tg) {
-  return tg.o_rec_bytes + tg.maxent * (int64_t)sizeof(GatRec);
-}
-
-// count -> scan -> scatter of the multi-coil gather plan (each exits at once on a cache
-// hit); the check kernel has cleared the counts
-static int gather_coils_plan(const Geom<float>& g, const TileGeom& tg, const void* omega, int* wsi,
-                             const CacheHdr* cache, hipStream_t s) {
-  dim3 gc((unsigned)ceil_div(g.M, 256), 1, 1);
-  gat_bin_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-  TKBN_LAUNCH_CHECK("gat_bin_count_kernel");
-  gat_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache);
-  TKBN_LAUNCH_CHECK("gat_plan_scan_kernel");
-  gat_bin_scatter_kernel<<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const float*>(omega), wsi, cache);
-  TKBN_LAUNCH_CHECK("gat_bin_scatter_kernel");
-  return TKBN_OK;
-}
-
-static int gather_coils_launch(const tkbn_interp_desc* d, const void* grid, const void* omega, void* kdata, void* ws,
-                               hipStream_t s) {
-  const Geom<float> g = make_geom<float>(d);
-  const TileGeom tg = make_gather_geom(d);
-  int* wsi = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
-  int st = TKBN_OK;
-  if (!plan_reused(d)) {
-    CacheHdr* cache = nullptr;
-    CacheZero z;
-    z.p = wsi + tg.o_count;
-    z.n = tg.NB;
-    st = plan_cache_begin(d, kPlanGatherCoils, tg.T, omega, ws, s, &cache, z);
-    if (st != TKBN_OK) return st;
-    st = gather_coils_plan(g, tg, omega, wsi, cache, s);
-    if (st != TKBN_OK) {
-      plan_cache_invalidate(ws, s);
-      return st;
-    }
-  } else {
-    st = plan_registry_require(d, kPlanGatherCoils, tg.T, ws);
-    if (st != TKBN_OK) return st;
-  }
-  const int nchunk = (int)ceil_div(g.B * g.C, kGatRows);
-  const int64_t groups = ceil_div(tg.maxwork * nchunk, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
-  TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_UNSUPPORTED, "too many coil rows x work items for one launch");
-  profile_before(s);
-  if (g.J[0] == 6 && g.J[1] == 6)
-    gather_coils_kernel<6><<<dim3((unsigned)groups), dim3(kGatThreads), 0, s>>>(
-        g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata), nchunk);
-  else
-    gather_coils_kernel<0><<<dim3((unsigned)groups), dim3(kGatThreads), 0, s>>>(
-        g, tg, static_cast<const float2*>(grid), wsi, static_cast<float2*>(kdata), nchunk);
-  profile_after(s);
-  TKBN_LAUNCH_CHECK("gather_coils_kernel");
-  note_kernel("gather_coils_kernel");
-  return TKBN_OK;
-}
-
 int64_t interp_fwd_workspace_bytes(const tkbn_interp_desc* d) {
-  if (gather_coils_eligible(d)) return plan_cache_bytes(d) + gather_workspace_bytes(make_gather_geom(d));
+  if (gather_coils_eligible(d)) {
+    SepGeom sg;
+    return plan_cache_bytes(d) + sep_workspace_bytes(make_sep_gather_geom(d, &sg), true);
+  }
   const TileGeom tg = make_tile_geom(d, false, true);
   const size_t lds = (size_t)tg.CC * tg.rcells * (d->dtype == TKBN_F32 ? 8 : 16);
   if (lds > (size_t)kLdsBudget) return 0;  // region too large for LDS: plain kernel
@@ -1844,7 +2050,7 @@ int64_t interp_fwd_workspace_bytes(const tkbn_interp_desc* d) {
 
 int interp_fwd_tiled(const tkbn_interp_desc* d, const void* grid, const void* omega, void* kdata, void* ws,
                      hipStream_t s) {
-  if (gather_coils_eligible(d)) return gather_coils_launch(d, grid, omega, kdata, ws, s);
+  if (gather_coils_eligible(d)) return gather_sep_launch(d, grid, omega, kdata, ws, s);
   if (d->dtype == TKBN_F32) return interp_fwd_tiled_t<float>(d, grid, omega, kdata, ws, s);
   return interp_fwd_tiled_t<double>(d, grid, omega, kdata, ws, s);
 }
@@ -1927,7 +2133,8 @@ int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mo
   const TileGeom tg = make_tile_geom(desc, det);
   int64_t n = plan_cache_bytes(desc) + spread_workspace_bytes(desc, tg);
   if (!det && coil_variant_eligible(desc)) {
-    const int64_t nc = plan_cache_bytes(desc) + coil_workspace_bytes(make_coil_tile_geom(desc));
+    SepGeom sg;
+    const int64_t nc = plan_cache_bytes(desc) + sep_workspace_bytes(make_sep_spread_geom(desc, &sg), false);
     n = nc > n ? nc : n;
   }
   return n;
