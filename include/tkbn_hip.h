@@ -37,6 +37,9 @@
  *   tkbn_spmv            <- _nufft/interp.py:13-85 spmat_interp / spmat_interp_adjoint
  *                           (the torch.mm products with the matrices of _nufft/spmat.py:10-105)
  *                           (ToepNufft's SENSE loop)
+ *   tkbn_batch_*         <- modules/kbnufft.py:125-228 / 294-410 called once per problem in a
+ *                           Python loop (one table_interp / table_interp_adjoint each):
+ *                           all problems of a batch in one launch per stage
  *   tkbn_interp_domega   <- no reference counterpart: the reference's autograd Functions
  *                           return None for the trajectory (_autograd/interp.py:130,178);
  *                           this gives d/d omega for both of them.
@@ -53,7 +56,7 @@
 extern "C" {
 #endif
 
-#define TKBN_ABI_VERSION 3
+#define TKBN_ABI_VERSION 4
 
 #if defined(__GNUC__)
 #define TKBN_API __attribute__((visibility("default")))
@@ -104,6 +107,16 @@ typedef struct tkbn_interp_desc {
   int64_t klength;
   int32_t omega_batched;
   int32_t flags; /* TKBN_PLAN_REUSE or 0 */
+  /* Tuning of the sample-lane binned kernels (fewer than 32 coil rows, or 1-D / 3-D,
+   * or complex128): 0 = the library's choice.  tile[d] is the grid-tile edge along
+   * dimension d (raised to numpoints[d] - 1, lowered to grid_size[d]; a tile whose
+   * one-row LDS staging exceeds 60 KiB is refused: prod_d (tile[d] + numpoints[d] - 1)
+   * complex values for the gather, prod_d tile[d] * 16 bytes for the spread); work_target is the number of (sample, tile) entries per work
+   * item of the atomic spread and per work item of the gather (>= 16).  Both are part
+   * of the plan: a workspace's plan is rebuilt when they change.  The multi-coil
+   * kernels and the unbinned gather ignore them. */
+  int32_t tile[3];
+  int32_t work_target;
 } tkbn_interp_desc;
 
 /*
@@ -176,6 +189,47 @@ TKBN_API int tkbn_interp_adj(const tkbn_interp_desc* desc, const void* kdata, co
 TKBN_API int64_t tkbn_interp_adj_noalloc_max_bytes(void);
 TKBN_API int tkbn_interp_adj_noalloc(const tkbn_interp_desc* desc, const void* kdata, const void* omega, void* grid,
                                      int32_t mode, void* stream);
+
+/*
+ * Batches of independent problems of uneven sizes (grids, sample counts, trajectories),
+ * replacing the reference's per-problem loop of module calls (modules/kbnufft.py:125-228,
+ * one table_interp / table_interp_adjoint per problem, _nufft/interp.py:315-403,587-726)
+ * with one launch per stage over all problems.
+ *
+ * Every problem is complex64, of the same ndim and numpoints class (all <= 6 or all
+ * <= 16), with an unbatched trajectory and fewer than 32 coil rows (the sample-lane
+ * kernels; larger problems take the multi-coil kernels, one call each).  The caller
+ * first makes one tkbn_interp_fwd and one tkbn_interp_adj call per problem with its
+ * two workspaces (this builds the plans; the forward needs its workspace), then:
+ *   tkbn_batch_create     describes the problems once: their descriptors (`adj_descs`, or
+ *                         NULL for `descs`, may differ from `descs` in the tuning fields
+ *                         only: the adjoint's tile and work target), pointers to each problem's forward
+ *                         input grid, forward output samples, adjoint input samples,
+ *                         adjoint output grid, trajectory and the two workspaces (fixed
+ *                         for the batch's life); `mode` as for tkbn_interp_adj;
+ *   tkbn_batch_write_table writes the batch's device table (tkbn_batch_table_bytes) into
+ *                         host memory; the caller copies it into device memory once;
+ *   tkbn_batch_interp_fwd / _adj run every problem's gather / spread on `stream` (the
+ *                         plans must still be those the workspaces were built with:
+ *                         checked on the host as for TKBN_PLAN_REUSE); a small caller
+ *                         workspace (tkbn_batch_workspace_bytes) holds the work-item
+ *                         offsets.  Results equal the per-problem calls'.
+ * TKBN_ERR_UNSUPPORTED from tkbn_batch_create names a problem outside those limits.
+ */
+typedef struct tkbn_batch tkbn_batch;
+TKBN_API int tkbn_batch_create(const tkbn_interp_desc* descs, const tkbn_interp_desc* adj_descs, int32_t nprob,
+                               int32_t mode, const void* const* grids_in,
+                               void* const* kdatas_out, const void* const* kdatas_in, void* const* grids_out,
+                               const void* const* omegas, void* const* fwd_workspaces, void* const* adj_workspaces,
+                               tkbn_batch** out);
+TKBN_API int tkbn_batch_destroy(tkbn_batch* batch);
+TKBN_API int64_t tkbn_batch_table_bytes(const tkbn_batch* batch);
+TKBN_API int tkbn_batch_write_table(const tkbn_batch* batch, void* host_dst);
+TKBN_API int64_t tkbn_batch_workspace_bytes(const tkbn_batch* batch);
+TKBN_API int tkbn_batch_interp_fwd(const tkbn_batch* batch, const void* table, void* workspace,
+                                   int64_t workspace_bytes, void* stream);
+TKBN_API int tkbn_batch_interp_adj(const tkbn_batch* batch, const void* table, void* workspace,
+                                   int64_t workspace_bytes, void* stream);
 
 /*
  * Trajectory gradient of a table interpolation:

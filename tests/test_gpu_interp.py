@@ -429,3 +429,42 @@ def test_multicoil_exception_entries_match_oracle(table):
     e_fb = rel_l2(kd[:, sel][..., idx].cpu(), ref_f[..., idx])
     print(f"[exceptions {table}] {int(bad.sum())} rounding samples; fwd {e_f:.2e} (those {e_fb:.2e}) adj {e_a:.2e}")
     assert e_f < C64_TOL and e_a < C64_TOL and e_fb < C64_TOL
+
+
+@pytest.mark.parametrize(
+    "im_size,grid,rows,M",
+    [((64, 60), (128, 120), 3, 20000), ((20, 18, 22), (40, 36, 44), 2, 20000), ((300,), (600,), 4, 5000)],
+)
+def test_tuned_tiles_and_targets_match_oracle(im_size, grid, rows, M):
+    """Every autotuner candidate (tile edge x work-item target, both directions, both
+    adjoint modes) gives the oracle's result; the plan is rebuilt when the tuning of a
+    holder changes and reused while it stays."""
+    from torchkbnufft_amd._nufft import autotune, interp as _interp
+
+    print(f"[case] tuned tiles grid={grid} rows={rows} M={M}", flush=True)
+    gen = torch.Generator().manual_seed(61)
+    d = len(grid)
+    om = create_ktraj(d, M, gen, torch.float32)
+    om[:, : M // 2] *= 0.05  # a dense centre: hot tiles cut into several work items
+    x = gi.cplx((1, rows) + grid, gen)
+    y = gi.cplx((1, rows, M), gen)
+    ob = tkbn.KbInterp(im_size=im_size, grid_size=grid, dtype=torch.complex64).to(DEV)
+    plan = tp.Plan(im_size, grid, dtype=torch.complex64)
+    ref_f = tp.kb_table_interp(plan, x, om)
+    ref_a = tp.kb_table_interp_adjoint(plan, y, om)
+    args = (om.to(DEV), ob._tables(), ob.n_shift, ob.numpoints, ob.table_oversamp, ob.offsets)
+    xd, yd = x.to(DEV), y.to(DEV)
+    jj = [6] * d
+    fwd_holder = _interp.PlanHolder(trusted=True)
+    for tile, target in autotune.candidates("fwd", grid, jj, False):
+        fwd_holder.set_tuning(tile, target)
+        for _ in range(2):  # build, then reuse
+            out = _interp.table_interp(xd, *args, plan=fwd_holder, binned=True).cpu()
+            assert rel_l2(out, ref_f) < C64_TOL, (tile, target)
+    for det in (False, True):
+        holder = _interp.PlanHolder(trusted=True)
+        for tile, target in autotune.candidates("adj", grid, jj, det):
+            holder.set_tuning(tile, target)
+            for _ in range(2):
+                out = _interp.table_interp_adjoint(yd, *args, grid, deterministic=det, plan=holder).cpu()
+                assert rel_l2(out, ref_a) < C64_TOL, (tile, target, det)

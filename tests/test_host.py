@@ -260,3 +260,82 @@ def test_noalloc_adjoint_refuses_oversized_and_flagged_calls_on_the_host():
     assert lib.tkbn_interp_adj_noalloc(ctypes.byref(d), 1, 1, 1, 0, None) == _lib.TKBN_ERR_INVALID
     assert b"flags" in lib.tkbn_last_error()
     assert lib.tkbn_interp_adj_noalloc(None, 1, 1, 1, 0, None) == _lib.TKBN_ERR_INVALID
+
+
+def test_tuning_fields_are_validated_on_the_host():
+    """tile / work_target overrides: an LDS-overflowing tile or a tiny target is refused
+    by the size queries (-1); a valid smaller tile gives a larger plan (more tiles)."""
+    lib = _lib.load()
+    d = _desc(ndim=2)
+    d.coils, d.klength = 4, 5000
+    for i in range(2):
+        d.grid_size[i] = 256
+    base = lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), _lib.TKBN_ADJ_ATOMIC)
+    assert base > 0
+    d.tile[0] = d.tile[1] = 8
+    small = lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), _lib.TKBN_ADJ_ATOMIC)
+    assert small > base
+    d.tile[0] = d.tile[1] = 64  # (64 + 5)^2 * 16 bytes > 60 KiB
+    assert lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), _lib.TKBN_ADJ_ATOMIC) == -1
+    assert lib.tkbn_interp_fwd_workspace_bytes(ctypes.byref(d)) == -1
+    assert b"LDS" in lib.tkbn_last_error()
+    d.tile[0] = d.tile[1] = 0
+    d.work_target = 8
+    assert lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), _lib.TKBN_ADJ_ATOMIC) == -1
+    d.work_target = 128  # smaller pieces: more work items in the plan
+    assert lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(d), _lib.TKBN_ADJ_ATOMIC) > base
+
+
+def _batch_ptrs(n, base=1 << 40):
+    arr = lambda k: (ctypes.c_void_p * n)(*[base + (k * n + p) * 4096 for p in range(n)])  # noqa: E731
+    return [arr(k) for k in range(7)]
+
+
+def test_batch_create_validates_problems_on_the_host():
+    """tkbn_batch_create only does host arithmetic (pointers are recorded, never read):
+    it refuses NULL pointers, multi-coil problems, mixed dimensions and forward / adjoint
+    descriptors that differ in more than tuning; a batch call whose plans were never
+    built is refused before anything is enqueued."""
+    lib = _lib.load()
+    n = 3
+    descs = (_lib.InterpDesc * n)()
+    for p in range(n):
+        d = _desc(ndim=2, M=3000 + 500 * p)
+        d.coils = 8
+        for i in range(2):
+            d.grid_size[i] = 128 + 64 * p
+        descs[p] = d
+    out = ctypes.c_void_p()
+    ptrs = _batch_ptrs(n)
+    assert lib.tkbn_batch_create(descs, None, n, 0, *ptrs, ctypes.byref(out)) == _lib.TKBN_OK
+    assert lib.tkbn_batch_table_bytes(out) > 0 and lib.tkbn_batch_table_bytes(out) % n == 0
+    assert lib.tkbn_batch_workspace_bytes(out) == 2 * (n + 1) * 4
+    host = (ctypes.c_ubyte * lib.tkbn_batch_table_bytes(out))()
+    assert lib.tkbn_batch_write_table(out, host) == _lib.TKBN_OK
+    ws = 1 << 41
+    assert lib.tkbn_batch_interp_fwd(out, 1 << 42, ws, 4096, None) == _lib.TKBN_ERR_INVALID
+    assert b"TKBN_PLAN_REUSE" in lib.tkbn_last_error()
+    assert lib.tkbn_batch_interp_adj(out, 1 << 42, ws, 4, None) == _lib.TKBN_ERR_WORKSPACE
+    assert lib.tkbn_batch_destroy(out) == _lib.TKBN_OK
+    # adjoint descriptors that differ only in tuning are accepted
+    adescs = (_lib.InterpDesc * n)()
+    for p in range(n):
+        adescs[p] = descs[p]
+        adescs[p].tile[0] = adescs[p].tile[1] = 16
+        adescs[p].work_target = 256
+    assert lib.tkbn_batch_create(descs, adescs, n, 0, *ptrs, ctypes.byref(out)) == _lib.TKBN_OK
+    lib.tkbn_batch_destroy(out)
+    adescs[1].klength += 1
+    assert lib.tkbn_batch_create(descs, adescs, n, 0, *ptrs, ctypes.byref(out)) == _lib.TKBN_ERR_INVALID
+    assert b"differ" in lib.tkbn_last_error()
+    # a NULL pointer, a 32-row problem, a 3-D problem among 2-D ones
+    bad = _batch_ptrs(n)
+    bad[3][1] = None
+    assert lib.tkbn_batch_create(descs, None, n, 0, *bad, ctypes.byref(out)) == _lib.TKBN_ERR_INVALID
+    descs[2].coils = 32
+    assert lib.tkbn_batch_create(descs, None, n, 0, *ptrs, ctypes.byref(out)) == _lib.TKBN_ERR_UNSUPPORTED
+    assert b"problem 2" in lib.tkbn_last_error()
+    descs[2].coils = 8
+    descs[1].ndim = 3
+    assert lib.tkbn_batch_create(descs, None, n, 0, *ptrs, ctypes.byref(out)) == _lib.TKBN_ERR_UNSUPPORTED
+    assert lib.tkbn_batch_create(descs, None, 0, 0, *ptrs, ctypes.byref(out)) == _lib.TKBN_ERR_INVALID

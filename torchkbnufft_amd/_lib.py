@@ -51,6 +51,8 @@ class InterpDesc(ctypes.Structure):
         ("klength", _c_i64),
         ("omega_batched", _c_i32),
         ("flags", _c_i32),
+        ("tile", _c_i32 * 3),
+        ("work_target", _c_i32),
     ]
 
 
@@ -69,6 +71,19 @@ _SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_i32, _c_vp],
     ),
+    "tkbn_batch_create": (
+        ctypes.c_int,
+        [ctypes.POINTER(InterpDesc), ctypes.POINTER(InterpDesc), _c_i32, _c_i32, ctypes.POINTER(_c_vp),
+         ctypes.POINTER(_c_vp),
+         ctypes.POINTER(_c_vp), ctypes.POINTER(_c_vp), ctypes.POINTER(_c_vp), ctypes.POINTER(_c_vp),
+         ctypes.POINTER(_c_vp), ctypes.POINTER(_c_vp)],
+    ),
+    "tkbn_batch_destroy": (ctypes.c_int, [_c_vp]),
+    "tkbn_batch_table_bytes": (_c_i64, [_c_vp]),
+    "tkbn_batch_write_table": (ctypes.c_int, [_c_vp, _c_vp]),
+    "tkbn_batch_workspace_bytes": (_c_i64, [_c_vp]),
+    "tkbn_batch_interp_fwd": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),
+    "tkbn_batch_interp_adj": (ctypes.c_int, [_c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),
     "tkbn_interp_domega": (ctypes.c_int, [ctypes.POINTER(InterpDesc), _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "tkbn_profile_next_kernel": (ctypes.c_int, [_c_vp, _c_vp]),
     "tkbn_last_interp_kernel": (ctypes.c_char_p, []),
@@ -108,7 +123,7 @@ _SIGNATURES = {
     ),
 }
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # bytes at the head of an interpolation workspace that hold the plan-cache header
 PLAN_CACHE_HEADER_BYTES = 256

@@ -129,10 +129,25 @@ class PlanHolder:
     otherwise the plan is checked on the device each call.  Use one stream per holder.
     """
 
-    def __init__(self, trusted: bool = False):
+    def __init__(self, trusted: bool = False, tile: Sequence[int] = (0, 0, 0), work_target: int = 0):
         self.ws: Optional[Tensor] = None
         self.built = False
         self.trusted = trusted
+        # tuning of the sample-lane binned kernels (0: the library's choice); a change
+        # makes the next call rebuild the plan
+        self.tile = tuple(int(v) for v in tile)
+        self.work_target = int(work_target)
+
+    def tune(self, desc: _lib.InterpDesc) -> None:
+        for i in range(3):
+            desc.tile[i] = self.tile[i] if i < len(self.tile) else 0
+        desc.work_target = self.work_target
+
+    def set_tuning(self, tile: Sequence[int], work_target: int) -> None:
+        tile = tuple(int(v) for v in tile)
+        if tile != self.tile or int(work_target) != self.work_target:
+            self.tile, self.work_target = tile, int(work_target)
+            self.built = False
 
     def get(self, like: Tensor, nbytes: int) -> Tensor:
         if self.ws is None or self.ws.numel() < nbytes or self.ws.device != like.device:
@@ -176,10 +191,12 @@ def table_interp(
     min_kspace_per_fork: int = 1024,
     out: Optional[Tensor] = None,
     plan: Optional[PlanHolder] = None,
+    binned: Optional[bool] = None,
 ) -> Tensor:
     """Interpolate gridded ``image`` (B, C, *K) to ``omega``; returns (B, C, M).
 
     ``out``: optional contiguous (B, C, M) tensor to write into.
+    ``binned``: force (True) or forbid (False) the binned gather; None: by problem size.
     """
     omega = _check_omega(omega, image.shape[0], "image")
     require_gpu(image, omega)
@@ -194,7 +211,9 @@ def table_interp(
     lib = _lib.load()
     ws, nws = None, 0
     # binned LDS-staged gather once there is enough work to amortise the plan launches
-    if omega.shape[-1] * B * C >= _BINNED_FWD_MIN_WORK:
+    if plan is not None:
+        plan.tune(desc)
+    if binned if binned is not None else omega.shape[-1] * B * C >= _BINNED_FWD_MIN_WORK:
         nws = int(lib.tkbn_interp_fwd_workspace_bytes(ctypes.byref(desc)))
         if nws > 0:
             if plan is not None:
@@ -250,6 +269,10 @@ def table_interp_adjoint(
     if ws_bytes < 0:
         _lib.check(_lib.TKBN_ERR_INVALID, "tkbn_interp_adj_workspace_bytes")
     if plan is not None:
+        plan.tune(desc)
+        ws_bytes = lib.tkbn_interp_adj_workspace_bytes(ctypes.byref(desc), mode)
+        if ws_bytes < 0:
+            _lib.check(_lib.TKBN_ERR_INVALID, "tkbn_interp_adj_workspace_bytes")
         ws = plan.get(data, int(ws_bytes))
         desc.flags = plan.flags()
     else:

@@ -6,30 +6,54 @@ problem.  Here
 
 * problems with the same image size share one operator (tables, apodisation), and
   their oversampled FFTs run as one fused launch over a stacked buffer;
-* every problem keeps its own trajectory plan in a workspace it owns; the plans are
-  built by the first call and then reused without any plan kernels (the trajectories
-  are fixed at construction), so a call only runs the gather / spread kernels of each
-  problem, writing into its slice of the stacked grid (no copies);
-* the problems of a size group are spread over a few side streams (fork after the
-  group's FFT, join before the next stage), so the many small interpolation launches
-  of uneven problems run concurrently instead of one after another;
+* every problem keeps its own trajectory plan in workspaces it owns, built once at
+  the first call (the trajectories are fixed at construction);
+* when every problem takes the sample-lane kernels (complex64, fewer than 32 coil
+  rows), the interpolations of ALL problems run as one gather launch and one spread
+  launch (``tkbn_batch_interp_fwd`` / ``_adj``): a device table holds each problem's
+  geometry, plans and buffers, and each block finds its (problem, work item) from
+  offsets computed on the device.  Otherwise each problem is its own launch, dealt
+  over a few side streams;
+* the tile edge and work-item target of each image size are picked by the
+  autotuner (``_nufft/autotune.py``) on one problem of that size;
 * ``capture()`` records the whole forward + adjoint pair of all problems in a HIP
-  graph (the fork/join becomes graph dependencies); ``replay()`` reruns it with one
-  launch from the host.
+  graph; ``replay()`` reruns it with one launch from the host.
 
-Each result is the same computation as the per-problem module call (same kernels,
-same plans), so it matches the reference within the same tolerance.
+Each result is the same computation as the per-problem module call (same weights,
+same sums; the float summation order of the spread may differ), so it matches the
+reference within the same tolerance.
 """
 from __future__ import annotations
 
-from typing import List, Optional, Sequence
+import ctypes
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import torch
 from torch import Tensor
 
+from . import _lib
+from ._nufft import autotune as _autotune
 from ._nufft import fft as _fft
 from ._nufft import interp as _interp
 from .modules import KbNufft
+
+
+class _BatchHandle:
+    """Owner of a ``tkbn_batch`` (the library object) and of its device table."""
+
+    def __init__(self, ptr: int, table: Tensor, workspace: Tensor, keep: list):
+        self.ptr = ptr
+        self.table = table
+        self.workspace = workspace
+        self.keep = keep  # tensors whose device pointers the table holds
+
+    def __del__(self):
+        if self.ptr:
+            try:
+                _lib.load().tkbn_batch_destroy(self.ptr)
+            except Exception:  # interpreter shutdown
+                pass
+            self.ptr = 0
 
 
 class BatchedKbNufft(torch.nn.Module):
@@ -41,8 +65,14 @@ class BatchedKbNufft(torch.nn.Module):
         coils: coil images per problem (each problem's image is (1, coils, *im_size)).
         numpoints, table_oversamp, kbwidth, dtype: as for ``KbNufft``.
         device: device of all buffers.
-        streams: side streams the problems of a group are dealt to (1: in order on the
-            caller's stream).  A problem always uses the same stream.
+        streams: side streams the problems of a group are dealt to when they run as
+            separate launches (1: in order on the caller's stream).
+        fused: one launch per stage for all problems (True: required, False: never,
+            None: when every problem qualifies, i.e. fewer than 32 coil rows).
+        autotune: pick the tile edge and work-item target per image size by timing
+            the candidates on one problem of that size (choices cached per geometry).
+        deterministic: adjoint mode of every problem (None: the global default when the
+            plans are built).
 
     Buffers (write inputs into them in place; outputs are overwritten):
         ``images[p]``  : (1, coils, *im_size_p) forward input
@@ -61,6 +91,9 @@ class BatchedKbNufft(torch.nn.Module):
         dtype: torch.dtype = torch.complex64,
         device: Optional[torch.device] = None,
         streams: int = 4,
+        fused: Optional[bool] = None,
+        autotune: bool = True,
+        deterministic: Optional[bool] = None,
     ):
         super().__init__()
         if len(im_sizes) != len(ktrajs):
@@ -118,7 +151,120 @@ class BatchedKbNufft(torch.nn.Module):
         self._nside = max(1, int(streams))
         self._side: List[torch.cuda.Stream] = []
         self._device = device
+        self._want_fused = fused
+        self._autotune = bool(autotune)
+        self._det = deterministic
+        self._built = False
+        self._batch: Optional[_BatchHandle] = None
+        # per image size: {"fwd": (tile, target), "adj": (tile, target)}
+        self.tuning: Dict[Tuple[int, ...], Dict[str, tuple]] = {}
 
+    # ------------------------------------------------------------------ set-up
+    def _interp_args(self, g: int):
+        op = self.ops[g]
+        return op._tables(), op.n_shift, op.numpoints, op.table_oversamp, op.offsets
+
+    def _tune(self, det: bool) -> None:
+        """Autotune each size group on its median-length problem (sample-lane kernels
+        only: from 32 coil rows the multi-coil kernels run, which take no tuning)."""
+        for g, members in enumerate(self.groups):
+            if self.coils >= 32:
+                break
+            tabs, n_shift, numpoints, L, offsets = self._interp_args(g)
+            p = sorted(members, key=lambda q: self.ktrajs[q].shape[1])[len(members) // 2]
+            grid = self._grid[g][:1]
+            fwd = _autotune.tune("fwd", grid, self.ktrajs[p], tabs, n_shift, numpoints, L, offsets, self._gs[g])
+            adj = _autotune.tune("adj", self.kdata[p], self.ktrajs[p], tabs, n_shift, numpoints, L, offsets,
+                                 self._gs[g], deterministic=det)
+            self.tuning[self.sizes[p]] = {"fwd": fwd, "adj": adj}
+            for q in members:
+                self._fwd_plans[q].set_tuning(*fwd)
+                self._adj_plans[q].set_tuning(*adj)
+
+    def _build(self) -> None:
+        """Build every problem's plans (one ordinary call per problem and direction into
+        scratch outputs), then the batch table when the problems qualify."""
+        det = _interp.deterministic_enabled() if self._det is None else bool(self._det)
+        self._det = det
+        if self._autotune:
+            self._tune(det)
+        fused = self._want_fused is not False and self.coils < 32
+        if self._want_fused and self.coils >= 32:
+            raise NotImplementedError("fused batches take fewer than 32 coil rows per problem")
+        for g, members in enumerate(self.groups):
+            tabs, n_shift, numpoints, L, offsets = self._interp_args(g)
+            gs = self._gs[g]
+            scratch_grid = torch.zeros((1, self.coils) + gs, dtype=self._grid[g].dtype, device=self._device)
+            for p in members:
+                _interp.table_interp(
+                    scratch_grid, self.ktrajs[p], tabs, n_shift, numpoints, L, offsets,
+                    out=torch.empty_like(self.kdata[p]), plan=self._fwd_plans[p], binned=True if fused else None,
+                )
+                _interp.table_interp_adjoint(
+                    self.kdata[p], self.ktrajs[p], tabs, n_shift, numpoints, L, offsets, gs, deterministic=det,
+                    out=scratch_grid, plan=self._adj_plans[p],
+                )
+        if fused:
+            self._batch = self._make_batch(det, required=bool(self._want_fused))
+        self._built = True
+
+    def _make_batch(self, det: bool, required: bool) -> Optional[_BatchHandle]:
+        lib = _lib.load()
+        n = len(self.sizes)
+        descs = (_lib.InterpDesc * n)()
+        ptrs = {k: (ctypes.c_void_p * n)() for k in ("gi", "ko", "ki", "go", "om", "wf", "wa")}
+        keep = []
+        for p in range(n):
+            g, slot = self.where[p]
+            tabs, n_shift, numpoints, L, _ = self._interp_args(g)
+            grid = self._grid[g][slot : slot + 1]
+            tabs = _interp._prep_tables(tabs, grid)
+            keep.extend(tabs)
+            d = _interp.make_desc(grid, self.ktrajs[p], tabs, n_shift, numpoints, L, self._gs[g], 1, self.coils)
+            self._fwd_plans[p].tune(d)
+            descs[p] = d
+            fw, aw = self._fwd_plans[p].ws, self._adj_plans[p].ws
+            if fw is None or aw is None:  # the forward took the unbinned gather
+                if required:
+                    raise NotImplementedError(f"problem {p} has no binned forward plan")
+                return None
+            ptrs["gi"][p] = grid.data_ptr()
+            ptrs["ko"][p] = self.kdata[p].data_ptr()
+            ptrs["ki"][p] = self.kdata[p].data_ptr()
+            ptrs["go"][p] = grid.data_ptr()
+            ptrs["om"][p] = self.ktrajs[p].data_ptr()
+            ptrs["wf"][p] = fw.data_ptr()
+            ptrs["wa"][p] = aw.data_ptr()
+        # the adjoint descriptors differ from the forward ones in their tuning only
+        adescs = (_lib.InterpDesc * n)()
+        for p in range(n):
+            adescs[p] = descs[p]
+            self._adj_plans[p].tune(adescs[p])
+        handle = ctypes.c_void_p()
+        mode = _lib.TKBN_ADJ_DETERMINISTIC if det else _lib.TKBN_ADJ_ATOMIC
+        st = lib.tkbn_batch_create(descs, adescs, n, mode, ptrs["gi"], ptrs["ko"], ptrs["ki"], ptrs["go"], ptrs["om"],
+                                   ptrs["wf"], ptrs["wa"], ctypes.byref(handle))
+        if st == _lib.TKBN_ERR_UNSUPPORTED and not required:
+            return None
+        _lib.check(st, "tkbn_batch_create")
+        try:
+            nbytes = int(lib.tkbn_batch_table_bytes(handle))
+            host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+            _lib.check(lib.tkbn_batch_write_table(handle, host.data_ptr()), "tkbn_batch_write_table")
+            table = host.to(self._device)
+            wsb = int(lib.tkbn_batch_workspace_bytes(handle))
+            workspace = torch.empty(max(wsb, 16), dtype=torch.uint8, device=self._device)
+        except Exception:
+            lib.tkbn_batch_destroy(handle)
+            raise
+        return _BatchHandle(handle.value, table, workspace, keep)
+
+    @property
+    def fused(self) -> bool:
+        """True when the interpolations run as one launch per stage (after the first call)."""
+        return self._batch is not None
+
+    # ------------------------------------------------------------------ per-problem launches
     def _streams(self) -> List[torch.cuda.Stream]:
         if self._nside > 1 and not self._side:
             self._side = [torch.cuda.Stream(device=self._device) for _ in range(self._nside)]
@@ -146,16 +292,25 @@ class BatchedKbNufft(torch.nn.Module):
             done.record(st)
             main.wait_event(done)
 
-    def _interp_args(self, g: int):
-        op = self.ops[g]
-        return op._tables(), op.n_shift, op.numpoints, op.table_oversamp, op.offsets
-
+    # ------------------------------------------------------------------ operators
     def forward(self) -> List[Tensor]:
         """kdata[p] = A_p images[p] for every problem."""
-        for g, members in enumerate(self.groups):
+        if not self._built:
+            self._build()
+        grids = []
+        for g in range(len(self.groups)):
             op = self.ops[g]
-            im, gs = self._im[g], self._gs[g]
-            grid = _fft.fft_grid_fwd(self._img[g], None, op.scaling_coef, im, gs, 1.0, out=self._grid[g])
+            grids.append(_fft.fft_grid_fwd(self._img[g], None, op.scaling_coef, self._im[g], self._gs[g], 1.0,
+                                           out=self._grid[g]))
+        if self._batch is not None:
+            b = self._batch
+            with torch.cuda.device(self._device):
+                st = _lib.load().tkbn_batch_interp_fwd(b.ptr, b.table.data_ptr(), b.workspace.data_ptr(),
+                                                       b.workspace.numel(), _lib.stream_of(b.table))
+            _lib.check(st, "tkbn_batch_interp_fwd")
+            return self.kdata
+        for g, members in enumerate(self.groups):
+            grid = grids[g]
             tabs, n_shift, numpoints, L, offsets = self._interp_args(g)
 
             def fwd(slot, p, grid=grid, tabs=tabs, n_shift=n_shift, numpoints=numpoints, L=L, offsets=offsets):
@@ -169,19 +324,29 @@ class BatchedKbNufft(torch.nn.Module):
 
     def adjoint(self) -> List[Tensor]:
         """adjoint_images[p] = A_p^H kdata[p] for every problem."""
+        if not self._built:
+            self._build()
+        if self._batch is not None:
+            b = self._batch
+            with torch.cuda.device(self._device):
+                st = _lib.load().tkbn_batch_interp_adj(b.ptr, b.table.data_ptr(), b.workspace.data_ptr(),
+                                                       b.workspace.numel(), _lib.stream_of(b.table))
+            _lib.check(st, "tkbn_batch_interp_adj")
         for g, members in enumerate(self.groups):
             op = self.ops[g]
             im, gs = self._im[g], self._gs[g]
-            tabs, n_shift, numpoints, L, offsets = self._interp_args(g)
             grid = self._grid[g]
+            if self._batch is None:
+                tabs, n_shift, numpoints, L, offsets = self._interp_args(g)
 
-            def adj(slot, p, grid=grid, tabs=tabs, n_shift=n_shift, numpoints=numpoints, L=L, offsets=offsets, gs=gs):
-                _interp.table_interp_adjoint(
-                    self.kdata[p], self.ktrajs[p], tabs, n_shift, numpoints, L, offsets, gs,
-                    out=grid[slot : slot + 1], plan=self._adj_plans[p],
-                )
+                def adj(slot, p, grid=grid, tabs=tabs, n_shift=n_shift, numpoints=numpoints, L=L, offsets=offsets,
+                        gs=gs):
+                    _interp.table_interp_adjoint(
+                        self.kdata[p], self.ktrajs[p], tabs, n_shift, numpoints, L, offsets, gs,
+                        deterministic=self._det, out=grid[slot : slot + 1], plan=self._adj_plans[p],
+                    )
 
-            self._fanout(members, adj)
+                self._fanout(members, adj)
             _fft.fft_grid_adj(grid, None, op.scaling_coef, im, 1.0, out=self._adj[g])
         return self.adjoint_images
 
@@ -192,7 +357,7 @@ class BatchedKbNufft(torch.nn.Module):
 
     def capture(self, warmup: int = 2) -> None:
         """Record ``pair()`` in a HIP graph on a side stream (plans are built by the
-        warm-up runs on that stream and reused by the graph)."""
+        warm-up runs, or earlier calls, and reused by the graph)."""
         self._stream = torch.cuda.Stream()
         self._stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(self._stream):

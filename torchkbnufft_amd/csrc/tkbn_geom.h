@@ -174,6 +174,26 @@ inline int validate_desc(const tkbn_interp_desc* d) {
   }
   TKBN_REQUIRE(G < (1LL << 40), TKBN_ERR_INVALID, "grid too large");
   TKBN_REQUIRE((d->flags & ~TKBN_PLAN_REUSE) == 0, TKBN_ERR_INVALID, "unknown flags 0x%x", d->flags);
+  TKBN_REQUIRE(d->work_target == 0 || (d->work_target >= 16 && d->work_target <= (1 << 24)), TKBN_ERR_INVALID,
+               "work_target %d not 0 or in 16..2^24", d->work_target);
+  // one row of an overridden tile must fit the kernels' LDS staging: the gather's
+  // region (tile + numpoints - 1 per dimension, one complex each) and the spread's two
+  // 8-byte accumulators per tile cell
+  int64_t region = d->dtype == TKBN_F32 ? 8 : 16, cells = 16;
+  bool tiled = false;
+  for (int i = 0; i < 3; ++i) {
+    TKBN_REQUIRE(d->tile[i] >= 0 && d->tile[i] <= 4096, TKBN_ERR_INVALID, "tile[%d] = %d", i, d->tile[i]);
+    if (i >= d->ndim) continue;
+    tiled = tiled || d->tile[i] > 0;
+    int64_t t = d->tile[i] > 0 ? d->tile[i] : 1;
+    if (t < d->numpoints[i] - 1) t = d->numpoints[i] - 1;
+    if (t > d->grid_size[i]) t = d->grid_size[i];
+    region *= t + d->numpoints[i] - 1;
+    cells *= t;
+  }
+  const int64_t stage = region > cells ? region : cells;
+  TKBN_REQUIRE(!tiled || stage <= 60 * 1024, TKBN_ERR_INVALID,
+               "tile override needs %lld bytes of LDS per row (at most 61440)", (long long)stage);
   return TKBN_OK;
 }
 

@@ -132,6 +132,7 @@ inline unsigned long long plan_cache_key(const tkbn_interp_desc* d, int kind, co
   h = fnv1a(h, d->n_shift, sizeof(d->n_shift));
   h = fnv1a(h, &d->klength, sizeof(d->klength));
   h = fnv1a(h, &d->omega_batched, sizeof(d->omega_batched));
+  h = fnv1a(h, &d->work_target, sizeof(d->work_target));
   return h | 1ULL;  // never 0, so a zeroed header never matches
 }
 

@@ -25,6 +25,7 @@
 // e = 60 - ceil(log2(2 * max|kdata| * candidates)), so no tile sum can overflow.
 #include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "tkbn_geom.h"
 #include "tkbn_plancache.h"
@@ -318,16 +319,14 @@ __global__ void __launch_bounds__(256) maxabs_kernel(const T* __restrict__ x, in
 // zero the cells of split tiles (fast mode), before their partial sums arrive
 // ---------------------------------------------------------------------------
 template <typename T, int D>
-__global__ void __launch_bounds__(256)
-    zero_hot_tiles_kernel(const Geom<T> g, const TileGeom tg, const int* __restrict__ ws, cplx<T>* __restrict__ grid) {
-  const int t = blockIdx.x;
-  const int64_t b = blockIdx.z;
+__device__ __forceinline__ void zero_hot_tiles_item(const Geom<T>& g, const TileGeom& tg, const int* __restrict__ ws,
+                                                    cplx<T>* __restrict__ grid, int t, int64_t yc, int64_t b) {
   const int* plan = ws + b * tg.plan_ints;
   const int* wstart = plan + tg.o_wstart;
   if (wstart[t + 1] - wstart[t] <= 1) return;
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t rowbase = g.batched ? b * g.C : 0;
-  const int64_t r0 = (int64_t)blockIdx.y * tg.CC;
+  const int64_t r0 = yc * tg.CC;
   const int64_t r1 = r0 + tg.CC < nrows ? r0 + tg.CC : nrows;
   int origin[3] = {0, 0, 0};
   for (int d = 0; d < D; ++d) origin[d] = ((t / tg.bstride[d]) % tg.nb[d]) * tg.T[d];
@@ -343,6 +342,12 @@ __global__ void __launch_bounds__(256)
     if (!in) continue;
     for (int64_t r = r0; r < r1; ++r) grid[(rowbase + r) * g.G + gi] = make_c<T>(0, 0);
   }
+}
+
+template <typename T, int D>
+__global__ void __launch_bounds__(256)
+    zero_hot_tiles_kernel(const Geom<T> g, const TileGeom tg, const int* __restrict__ ws, cplx<T>* __restrict__ grid) {
+  zero_hot_tiles_item<T, D>(g, tg, ws, grid, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -367,11 +372,14 @@ struct AccOf<T, true> {
 // than float ones; the sum is also order independent).  It exits when the max|kdata|
 // slot says a non-finite value is present; the float kernel launched with
 // `fallback` = 1 runs only in that case, so NaN and inf still propagate.
+// One work item: tile piece wi, chunk yc of tg.CC coil rows, per-batch plan b (blockIdx
+// of spread_tiles_kernel, or a batch kernel's decoded item).
 template <typename T, bool FIXED, int D, int JM>
-__global__ void __launch_bounds__(256)
-    spread_tiles_kernel(const Geom<T> g, const TileGeom tg, const cplx<T>* __restrict__ kdata,
-                        const T* __restrict__ omega, const int* __restrict__ ws, const void* __restrict__ maxabs_slot,
-                        cplx<T>* __restrict__ grid, const int fallback) {
+__device__ __forceinline__ void spread_tiles_item(const Geom<T>& g, const TileGeom& tg, const cplx<T>* __restrict__ kdata,
+                                                  const T* __restrict__ omega, const int* __restrict__ ws,
+                                                  const void* __restrict__ maxabs_slot, cplx<T>* __restrict__ grid,
+                                                  const int fallback, int64_t wi, int64_t yc, int64_t b,
+                                                  unsigned char* smem_raw) {
   using C2 = cplx<T>;
   using A = typename AccOf<T, FIXED>::type;
   T amax = 0;
@@ -379,14 +387,9 @@ __global__ void __launch_bounds__(256)
     memcpy(&amax, maxabs_slot, sizeof(T));
     if (FIXED != (bool)isfinite(amax)) return;  // the other kernel of the pair handles this call
   }
-  // 16-byte aligned: 8-byte LDS atomics (double / int64 accumulators) need natural
-  // alignment, and the static LDS above this array is not a multiple of 8 bytes
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   A* acc = reinterpret_cast<A*>(smem_raw);
 
-  const int64_t b = blockIdx.z;
   const int* plan = ws + b * tg.plan_ints;
-  const int64_t wi = blockIdx.x;
   if (wi >= plan[tg.o_nwork]) return;
   const int* wstart = plan + tg.o_wstart;
   const int t = plan[tg.o_work + wi];
@@ -395,7 +398,7 @@ __global__ void __launch_bounds__(256)
 
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t rowbase = g.batched ? b * g.C : 0;
-  const int64_t r0 = (int64_t)blockIdx.y * tg.CC;
+  const int64_t r0 = yc * tg.CC;
   const int ncc = (int)((r0 + tg.CC < nrows ? r0 + tg.CC : nrows) - r0);
 
   int origin[3] = {0, 0, 0};
@@ -518,6 +521,18 @@ __global__ void __launch_bounds__(256)
       unsafeAtomicAdd(&dst->y, val.y);
     }
   }
+}
+
+template <typename T, bool FIXED, int D, int JM>
+__global__ void __launch_bounds__(256)
+    spread_tiles_kernel(const Geom<T> g, const TileGeom tg, const cplx<T>* __restrict__ kdata,
+                        const T* __restrict__ omega, const int* __restrict__ ws, const void* __restrict__ maxabs_slot,
+                        cplx<T>* __restrict__ grid, const int fallback) {
+  // 16-byte aligned: 8-byte LDS atomics (double / int64 accumulators) need natural
+  // alignment, and the static LDS above this array is not a multiple of 8 bytes
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  spread_tiles_item<T, FIXED, D, JM>(g, tg, kdata, omega, ws, maxabs_slot, grid, fallback, blockIdx.x, blockIdx.y,
+                                     blockIdx.z, smem_raw);
 }
 
 // ---------------------------------------------------------------------------
@@ -654,16 +669,16 @@ __device__ __forceinline__ int64_t xcd_work_item(int64_t b) {
 //   (wrapped) for the chunk's rows, then every sample based in the tile reads its
 //   J^d neighbourhood from LDS.
 // ---------------------------------------------------------------------------
+// One work item: tile piece wi, chunk yc of tg.CC coil rows, per-batch plan b (blockIdx
+// of interp_fwd_tiled_kernel, or a batch kernel's decoded item).
 template <typename T, int D, int JM>
-__global__ void __launch_bounds__(256)
-    interp_fwd_tiled_kernel(const Geom<T> g, const TileGeom tg, const cplx<T>* __restrict__ grid,
-                            const T* __restrict__ omega, const int* __restrict__ ws, cplx<T>* __restrict__ kdata) {
+__device__ __forceinline__ void fwd_tiled_item(const Geom<T>& g, const TileGeom& tg, const cplx<T>* __restrict__ grid,
+                                               const T* __restrict__ omega, const int* __restrict__ ws,
+                                               cplx<T>* __restrict__ kdata, int64_t wi, int64_t yc, int64_t b,
+                                               unsigned char* smem_raw) {
   using C2 = cplx<T>;
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C2* lds = reinterpret_cast<C2*>(smem_raw);
-  const int64_t b = blockIdx.z;
   const int* plan = ws + b * tg.plan_ints;
-  const int64_t wi = blockIdx.x;
   if (wi >= plan[tg.o_nwork]) return;
   const int* wstart = plan + tg.o_wstart;
   const int t = plan[tg.o_work + wi];
@@ -675,7 +690,7 @@ __global__ void __launch_bounds__(256)
 
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t rowbase = g.batched ? b * g.C : 0;
-  const int64_t r0 = (int64_t)blockIdx.y * tg.CC;
+  const int64_t r0 = yc * tg.CC;
   const int ncc = (int)((r0 + tg.CC < nrows ? r0 + tg.CC : nrows) - r0);
 
   int origin[3] = {0, 0, 0};
@@ -762,6 +777,14 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+template <typename T, int D, int JM>
+__global__ void __launch_bounds__(256)
+    interp_fwd_tiled_kernel(const Geom<T> g, const TileGeom tg, const cplx<T>* __restrict__ grid,
+                            const T* __restrict__ omega, const int* __restrict__ ws, cplx<T>* __restrict__ kdata) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  fwd_tiled_item<T, D, JM>(g, tg, grid, omega, ws, kdata, blockIdx.x, blockIdx.y, blockIdx.z, smem_raw);
+}
+
 // ---------------------------------------------------------------------------
 // separable multi-coil kernels (2-D, complex64, J <= 6)
 //   The reference's table for dimension d holds t[i] = KB(u) exp(-i pi (N-1) u / K)
@@ -780,6 +803,7 @@ __global__ void __launch_bounds__(256)
 struct SepGeom {
   int L[2];            // table oversampling
   int64_t o_rtab[2];   // int offsets of the real tables in the plan
+  int64_t o_ptab[2];   // int offsets of the cell phases P_d(k), k < K_d (float2) in the plan
   int64_t o_nm1;       // int offset of N_d - 1 (2 ints), found from the tables by the plan
 };
 
@@ -803,6 +827,19 @@ __device__ __forceinline__ float2 pi_phase(int64_t a, int64_t q) {
   float s, c;
   sincospif(x, &s, &c);
   return make_float2(c, s);
+}
+
+// P(p) = exp(i pi (N - 1) p / K) for a cell coordinate p >= 0 that may run past the
+// grid's end (the gather's staged halo): P(p) = P(p - K) (-1)^(N - 1), from the plan's
+// table of P(k), k < K.
+__device__ __forceinline__ float2 sep_cell_phase(const float2* P, int K, int nm1, int p) {
+  bool flip = false;
+  while (p >= K) {
+    p -= K;
+    flip = flip != ((nm1 & 1) != 0);
+  }
+  const float2 v = P[p];
+  return flip ? make_float2(-v.x, -v.y) : v;
 }
 
 // Plan record of one (sample, tile) entry (64 bytes).  Forward: the sample's base cell
@@ -908,6 +945,12 @@ __device__ __forceinline__ void sep_table_prep(const Geom<float>& g, const SepGe
     const float im = t.x * p.y + t.y * p.x;
     reinterpret_cast<float*>(plan + sg.o_rtab[d])[m] = re;
     if (!(fabsf(im) <= 0x1p-19f * (fabsf(t.x) + fabsf(t.y)))) atomicOr(flag, 1);
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)g.K[0] + g.K[1];
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int d = i < g.K[0] ? 0 : 1;
+    const int k = (int)(d == 0 ? i : i - g.K[0]);
+    reinterpret_cast<float2*>(plan + sg.o_ptab[d])[k] = pi_phase((int64_t)nm1[d] * k, g.K[d]);
   }
 }
 
@@ -1147,6 +1190,8 @@ __global__ void __launch_bounds__(kGatThreads)
   const int o0 = ((t / tg.bstride[0]) % tg.nb[0]) * kGatT0;
   const int o1 = ((t / tg.bstride[1]) % tg.nb[1]) * kGatT1;
   const int nm1[2] = {plan[sg.o_nm1], plan[sg.o_nm1 + 1]};
+  const float2* P0 = reinterpret_cast<const float2*>(plan + sg.o_ptab[0]);
+  const float2* P1 = reinterpret_cast<const float2*>(plan + sg.o_ptab[1]);
 
   // Stage: thread (coil row sr, region column sl1) of the first 16 x 21 threads loads
   // that column's 21 region rows (consecutive threads read consecutive grid columns;
@@ -1170,7 +1215,7 @@ __global__ void __launch_bounds__(kGatThreads)
   } else {
     for (int c = (int)threadIdx.x - kStageThreads; c < kGatCells; c += kGatThreads - kStageThreads) {
       const int l0 = c / kGatR1, l1 = c - l0 * kGatR1;
-      cph[c] = cmul(pi_phase((int64_t)nm1[0] * (o0 + l0), g.K[0]), pi_phase((int64_t)nm1[1] * (o1 + l1), g.K[1]));
+      cph[c] = cmul(sep_cell_phase(P0, g.K[0], nm1[0], o0 + l0), sep_cell_phase(P1, g.K[1], nm1[1], o1 + l1));
     }
   }
   lds_barrier();
@@ -1289,13 +1334,14 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
   const SepRec* recs = reinterpret_cast<const SepRec*>(reinterpret_cast<const char*>(ws) + tg.o_rec_bytes) + cstart;
   const int J0 = g.J[0];
   const int wrow = wave * kRegRowsPerWave;
-  const int nm1[2] = {plan[sg.o_nm1], plan[sg.o_nm1 + 1]};
 
+  // cells past the grid's end are never flushed: any phase will do there
   if (threadIdx.x < kRegT0) {
-    const float2 p = pi_phase((int64_t)nm1[0] * (o0 + (int)threadIdx.x), g.K[0]);
+    const float2 p = reinterpret_cast<const float2*>(plan + sg.o_ptab[0])[min(o0 + (int)threadIdx.x, g.K[0] - 1)];
     rph[threadIdx.x] = make_float2(p.x, -p.y);
   } else if (threadIdx.x < kRegT0 + kRegT1) {
-    const float2 p = pi_phase((int64_t)nm1[1] * (o1 + (int)threadIdx.x - kRegT0), g.K[1]);
+    const float2 p =
+        reinterpret_cast<const float2*>(plan + sg.o_ptab[1])[min(o1 + (int)threadIdx.x - kRegT0, g.K[1] - 1)];
     cphs[threadIdx.x - kRegT0] = make_float2(p.x, -p.y);
   }
   lds_barrier();  // the flush reads them even when the tile has no entries
@@ -1518,8 +1564,10 @@ static TileGeom make_tile_geom(const tkbn_interp_desc* d, bool det, bool fwd = f
     tg.K[i] = 1;
     tg.J[i] = 1;
   }
+  bool tuned = false;  // caller's tile edges (validated to fit one row in LDS)
+  for (int i = 0; i < D; ++i) tuned = tuned || d->tile[i] > 0;
   for (int i = 0; i < D; ++i) {
-    int T = shape[i];
+    int T = tuned ? (d->tile[i] > 0 ? d->tile[i] : 1) : shape[i];
     const int need = (int)d->numpoints[i] - 1;
     if (T < need) T = need;
     if (T > d->grid_size[i]) T = (int)d->grid_size[i];
@@ -1559,7 +1607,7 @@ static TileGeom make_tile_geom(const tkbn_interp_desc* d, bool det, bool fwd = f
   tg.CC = cc;
   // entries per work item: hot tiles are cut into pieces of `target` (measured: 512 for
   // the spread, against 1024 / 2048, is 10-20 % faster on a 12-coil 256^2 spiral)
-  tg.target = det ? 0x3fffffff : (fwd ? 1024 : 512);
+  tg.target = det ? 0x3fffffff : (d->work_target > 0 ? d->work_target : (fwd ? 1024 : 512));
   const int64_t M = d->klength;
   // a sample reaches at most maxt_d tiles along dimension d: 1 with a single tile,
   // 2 when every tile is at least J-1 long, 3 when the last tile is shorter
@@ -1743,6 +1791,10 @@ constexpr int kPlanSepSpread = 7;
 static void sep_finish_layout(TileGeom& tg, SepGeom& sg, int64_t o, const tkbn_interp_desc* d) {
   sg.o_nm1 = o;
   o += 2;
+  for (int i = 0; i < 2; ++i) {
+    sg.o_ptab[i] = o;
+    o += 2 * d->grid_size[i];
+  }
   for (int i = 0; i < 2; ++i) {
     sg.o_rtab[i] = o;
     o += d->numpoints[i] * d->table_oversamp[i] + 1;
@@ -2055,6 +2107,375 @@ int interp_fwd_tiled(const tkbn_interp_desc* d, const void* grid, const void* om
   return interp_fwd_tiled_t<double>(d, grid, omega, kdata, ws, s);
 }
 
+
+// ---------------------------------------------------------------------------
+// batches of uneven problems (complex64, sample-lane paths): one launch per stage
+//   The caller builds every problem's plans with ordinary tkbn_interp_fwd / _adj calls
+//   (the same workspaces), then describes the problems once (tkbn_batch_create); the
+//   library writes a table of per-problem records (geometry, tile plans, pointers) that
+//   the caller copies to device memory.  Each batched call then computes the work-item
+//   offsets of all problems on the device (their plans' work counts) and runs one launch
+//   per stage: block -> (problem, work item) by a binary search over the offsets, and
+//   the same per-item code as the single-problem kernels.
+// ---------------------------------------------------------------------------
+struct BatchProb {
+  Geom<float> g;
+  TileGeom tf, ta;         // forward / adjoint sample-lane tile plans
+  const float2* grid_in;   // forward input grid
+  float2* kdata_out;       // forward output
+  const float2* kdata_in;  // adjoint input
+  float2* grid_out;        // adjoint output grid (fully overwritten)
+  const float* omega;
+  const int* wf;           // forward plan (past the workspace's cache header)
+  const int* wa;           // adjoint plan
+  void* slot;              // adjoint max|re|,|im| slot (8 bytes, cleared each call)
+  int64_t yf, ya;          // coil-row chunks per work item, forward / adjoint
+  int64_t zoff;            // first zero-hot block of this problem (host prefix of NB * ya)
+  int64_t nom;             // trajectory values (ndim * klength)
+};
+
+// Offsets of the problems' work items: off_f[p] = sum_{q < p} nwork_f(q) * yf(q) (and
+// off_a the same for the adjoint), off[nprob] the totals; clears every adjoint slot.
+__global__ void __launch_bounds__(1024) batch_offsets_kernel(const BatchProb* __restrict__ probs, int nprob,
+                                                             int* __restrict__ off_f, int* __restrict__ off_a,
+                                                             int clear_slots) {
+  __shared__ int sbuf[1024];
+  __shared__ int cf[1024], ca[1024];
+  __shared__ int base[2];
+  if (threadIdx.x == 0) base[0] = base[1] = 0;
+  for (int p0 = 0; p0 < nprob; p0 += 1024) {
+    const int n = min(1024, nprob - p0);
+    if ((int)threadIdx.x < n) {
+      const BatchProb& q = probs[p0 + threadIdx.x];
+      cf[threadIdx.x] = q.wf[q.tf.o_nwork] * (int)q.yf;
+      ca[threadIdx.x] = q.wa[q.ta.o_nwork] * (int)q.ya;
+      if (clear_slots) *static_cast<unsigned long long*>(q.slot) = 0ull;
+    }
+    __syncthreads();
+    block_exclusive_scan(cf, off_f + p0, n, sbuf);  // off_f[p0 .. p0 + n]
+    block_exclusive_scan(ca, off_a + p0, n, sbuf);
+    for (int i = threadIdx.x; i <= n; i += blockDim.x) {
+      off_f[p0 + i] += base[0];
+      off_a[p0 + i] += base[1];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      base[0] = off_f[p0 + n];
+      base[1] = off_a[p0 + n];
+    }
+    __syncthreads();
+  }
+}
+
+// largest p with off[p] <= item (off[0] = 0 <= item < off[nprob])
+__device__ __forceinline__ int batch_find(const int* off, int nprob, int item) {
+  int lo = 0, hi = nprob;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int D, int JM>
+__global__ void __launch_bounds__(256)
+    batch_fwd_kernel(const BatchProb* __restrict__ probs, int nprob, const int* __restrict__ off) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int item = blockIdx.x;
+  if (item >= off[nprob]) return;
+  const int p = batch_find(off, nprob, item);
+  const BatchProb& q = probs[p];
+  const int li = item - off[p];
+  const int64_t wi = li / q.yf, yc = li - wi * q.yf;
+  fwd_tiled_item<float, D, JM>(q.g, q.tf, q.grid_in, q.omega, q.wf, q.kdata_out, wi, yc, 0, smem_raw);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) batch_zero_hot_kernel(const BatchProb* __restrict__ probs, int nprob) {
+  const int64_t blk = blockIdx.x;
+  int lo = 0, hi = nprob;  // largest p with zoff(p) <= blk
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (probs[mid].zoff <= blk) lo = mid;
+    else hi = mid;
+  }
+  const BatchProb& q = probs[lo];
+  const int64_t li = blk - q.zoff;
+  const int64_t t = li / q.ya, yc = li - t * q.ya;
+  if (t >= q.ta.NB) return;
+  zero_hot_tiles_item<float, D>(q.g, q.ta, q.wa, q.grid_out, (int)t, yc, 0);
+}
+
+// max |re|, |im| of each problem's samples (blockIdx.y = problem), +inf for a non-finite
+// sample or trajectory value, into the problem's slot
+__global__ void __launch_bounds__(256) batch_maxabs_kernel(const BatchProb* __restrict__ probs) {
+  const BatchProb& q = probs[blockIdx.y];
+  const float* x = reinterpret_cast<const float*>(q.kdata_in);
+  const int64_t n = q.g.B * q.g.C * q.g.M * 2;
+  float mx = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = fabsf(x[i]);
+    mx = isfinite(v) ? (v > mx ? v : mx) : INFINITY;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < q.nom; i += (int64_t)gridDim.x * blockDim.x)
+    if (!isfinite(q.omega[i])) mx = INFINITY;
+  for (int o = 32; o > 0; o >>= 1) {
+    const float v = __shfl_down(mx, o, 64);
+    mx = v > mx ? v : mx;
+  }
+  __shared__ float wmax[4];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) mx = wmax[w] > mx ? wmax[w] : mx;
+    unsigned int bits;
+    memcpy(&bits, &mx, sizeof(bits));
+    atomicMax(static_cast<unsigned int*>(q.slot), bits);
+  }
+}
+
+template <bool FIXED, int D, int JM>
+__global__ void __launch_bounds__(256)
+    batch_spread_kernel(const BatchProb* __restrict__ probs, int nprob, const int* __restrict__ off, int fallback) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int item = blockIdx.x;
+  if (item >= off[nprob]) return;
+  const int p = batch_find(off, nprob, item);
+  const BatchProb& q = probs[p];
+  const int li = item - off[p];
+  const int64_t wi = li / q.ya, yc = li - wi * q.ya;
+  spread_tiles_item<float, FIXED, D, JM>(q.g, q.ta, q.kdata_in, q.omega, q.wa, q.slot, q.grid_out, fallback, wi, yc, 0,
+                                         smem_raw);
+}
+
+}  // namespace tkbn
+
+struct tkbn_batch {
+  int nprob = 0, ndim = 0, jm = 6, det = 0;
+  std::vector<tkbn::BatchProb> probs;
+  std::vector<tkbn_interp_desc> descs_f, descs_a;
+  std::vector<const void*> wsf, wsa;
+  size_t lds_f = 0, lds_a = 0;
+  int64_t bound_f = 0, bound_a = 0, zblocks = 0, maxabs_blocks = 1;
+};
+
+namespace tkbn {
+
+// the batch's plans must be the ones its workspaces hold (same check as TKBN_PLAN_REUSE)
+static int batch_plans_current(const tkbn_batch* b) {
+  for (int p = 0; p < b->nprob; ++p) {
+    int st = plan_registry_require(&b->descs_f[p], kPlanFwdTiles, b->probs[p].tf.T, b->wsf[p]);
+    if (st != TKBN_OK) return st;
+    st = plan_registry_require(&b->descs_a[p], kPlanSpreadTiles + b->det, b->probs[p].ta.T, b->wsa[p]);
+    if (st != TKBN_OK) return st;
+  }
+  return TKBN_OK;
+}
+
+// the two descriptors of a problem agree on everything but the tuning fields
+static bool same_geometry(const tkbn_interp_desc* a, const tkbn_interp_desc* b) {
+  if (a->dtype != b->dtype || a->ndim != b->ndim || a->batch != b->batch || a->coils != b->coils ||
+      a->klength != b->klength || a->omega_batched != b->omega_batched)
+    return false;
+  for (int i = 0; i < a->ndim; ++i)
+    if (a->grid_size[i] != b->grid_size[i] || a->numpoints[i] != b->numpoints[i] ||
+        a->table_oversamp[i] != b->table_oversamp[i] || a->n_shift[i] != b->n_shift[i] || a->tables[i] != b->tables[i])
+      return false;
+  return true;
+}
+
+}  // namespace tkbn
+
+extern "C" {
+
+int tkbn_batch_create(const tkbn_interp_desc* descs, const tkbn_interp_desc* adj_descs, int32_t nprob, int32_t mode,
+                      const void* const* grids_in,
+                      void* const* kdatas_out, const void* const* kdatas_in, void* const* grids_out,
+                      const void* const* omegas, void* const* fwd_workspaces, void* const* adj_workspaces,
+                      tkbn_batch** out) {
+  using namespace tkbn;
+  TKBN_REQUIRE(out != nullptr && descs != nullptr && nprob >= 1, TKBN_ERR_INVALID, "tkbn_batch_create: no problems");
+  TKBN_REQUIRE(mode == TKBN_ADJ_ATOMIC || mode == TKBN_ADJ_DETERMINISTIC, TKBN_ERR_INVALID, "bad adjoint mode %d", mode);
+  *out = nullptr;
+  tkbn_batch* b = new tkbn_batch();
+  b->nprob = nprob;
+  b->det = mode == TKBN_ADJ_DETERMINISTIC ? 1 : 0;
+  b->ndim = descs[0].ndim;
+  b->jm = max_numpoints(&descs[0]) <= 6 ? 6 : 16;
+  int64_t zoff = 0;
+  for (int p = 0; p < nprob; ++p) {
+    const tkbn_interp_desc* d = &descs[p];
+    const tkbn_interp_desc* da = adj_descs ? &adj_descs[p] : d;
+    int st = validate_desc(d);
+    if (st == TKBN_OK) st = validate_desc(da);
+    if (st == TKBN_OK && !same_geometry(d, da)) {
+      set_error("tkbn_batch_create: problem %d: forward and adjoint descriptors differ in more than tuning", p);
+      st = TKBN_ERR_INVALID;
+    }
+    if (st == TKBN_OK && (d->dtype != TKBN_F32 || d->ndim != b->ndim || d->omega_batched ||
+                          (max_numpoints(d) <= 6 ? 6 : 16) != b->jm)) {
+      set_error("tkbn_batch_create: problem %d: batches take complex64 problems of one dimension and numpoints "
+                "class with unbatched trajectories", p);
+      st = TKBN_ERR_UNSUPPORTED;
+    }
+    if (st == TKBN_OK && (gather_coils_eligible(d) || interp_fwd_workspace_bytes(d) <= 0 ||
+                          (!b->det && coil_variant_eligible(da)))) {
+      set_error("tkbn_batch_create: problem %d takes the multi-coil or unbinned path; batches run the sample-lane "
+                "kernels only (fewer than 32 coil rows)", p);
+      st = TKBN_ERR_UNSUPPORTED;
+    }
+    if (st == TKBN_OK && !(grids_in && grids_in[p] && kdatas_out && kdatas_out[p] && kdatas_in && kdatas_in[p] &&
+                           grids_out && grids_out[p] && omegas && omegas[p] && fwd_workspaces &&
+                           fwd_workspaces[p] && adj_workspaces && adj_workspaces[p])) {
+      set_error("tkbn_batch_create: problem %d: NULL pointer", p);
+      st = TKBN_ERR_INVALID;
+    }
+    if (st != TKBN_OK) {
+      delete b;
+      return st;
+    }
+    BatchProb q{};
+    q.g = make_geom<float>(d);
+    q.tf = make_tile_geom(d, false, true);
+    q.ta = make_tile_geom(da, b->det != 0);
+    q.grid_in = static_cast<const float2*>(grids_in[p]);
+    q.kdata_out = static_cast<float2*>(kdatas_out[p]);
+    q.kdata_in = static_cast<const float2*>(kdatas_in[p]);
+    q.grid_out = static_cast<float2*>(grids_out[p]);
+    q.omega = static_cast<const float*>(omegas[p]);
+    const int64_t hdr = plan_cache_bytes(d);
+    q.wf = reinterpret_cast<const int*>(static_cast<const char*>(fwd_workspaces[p]) + hdr);
+    q.wa = reinterpret_cast<const int*>(static_cast<const char*>(adj_workspaces[p]) + hdr);
+    q.slot = static_cast<char*>(adj_workspaces[p]) + hdr + q.ta.plan_ints * 4;
+    const int64_t nrows = d->batch * d->coils;
+    q.yf = ceil_div(nrows, q.tf.CC);
+    q.ya = ceil_div(nrows, q.ta.CC);
+    q.zoff = zoff;
+    q.nom = (int64_t)d->ndim * d->klength;
+    zoff += (int64_t)q.ta.NB * q.ya;
+    b->lds_f = std::max(b->lds_f, (size_t)q.tf.CC * q.tf.rcells * sizeof(float2));
+    b->lds_a = std::max(b->lds_a, spread_lds_bytes(da, q.ta, b->det != 0));
+    b->bound_f += q.tf.maxwork * q.yf;
+    b->bound_a += q.ta.maxwork * q.ya;
+    b->maxabs_blocks = std::max<int64_t>(b->maxabs_blocks, std::min<int64_t>(ceil_div(nrows * d->klength * 2, 4096), 64));
+    b->probs.push_back(q);
+    b->descs_f.push_back(*d);
+    b->descs_f.back().flags = 0;
+    b->descs_a.push_back(*da);
+    b->descs_a.back().flags = 0;
+    b->wsf.push_back(fwd_workspaces[p]);
+    b->wsa.push_back(adj_workspaces[p]);
+  }
+  b->zblocks = zoff;
+  if (b->bound_f >= (1LL << 31) || b->bound_a >= (1LL << 31) || b->zblocks >= (1LL << 31)) {
+    set_error("tkbn_batch_create: too many work items for one launch");
+    delete b;
+    return TKBN_ERR_UNSUPPORTED;
+  }
+  *out = b;
+  return TKBN_OK;
+}
+
+int tkbn_batch_destroy(tkbn_batch* b) {
+  delete b;
+  return TKBN_OK;
+}
+
+int64_t tkbn_batch_table_bytes(const tkbn_batch* b) {
+  return b ? (int64_t)(b->probs.size() * sizeof(tkbn::BatchProb)) : -1;
+}
+
+int tkbn_batch_write_table(const tkbn_batch* b, void* host_dst) {
+  TKBN_REQUIRE(b != nullptr && host_dst != nullptr, TKBN_ERR_INVALID, "tkbn_batch_write_table: NULL argument");
+  memcpy(host_dst, b->probs.data(), b->probs.size() * sizeof(tkbn::BatchProb));
+  return TKBN_OK;
+}
+
+int64_t tkbn_batch_workspace_bytes(const tkbn_batch* b) {
+  return b ? (int64_t)(2 * (b->nprob + 1) * sizeof(int)) : -1;
+}
+
+int tkbn_batch_interp_fwd(const tkbn_batch* b, const void* table, void* workspace, int64_t workspace_bytes,
+                          void* stream) {
+  using namespace tkbn;
+  TKBN_REQUIRE(b != nullptr && table != nullptr && workspace != nullptr, TKBN_ERR_INVALID, "tkbn_batch_interp_fwd: NULL argument");
+  TKBN_REQUIRE(workspace_bytes >= tkbn_batch_workspace_bytes(b), TKBN_ERR_WORKSPACE, "batch workspace too small");
+  int st = batch_plans_current(b);
+  if (st != TKBN_OK) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const auto* probs = static_cast<const BatchProb*>(table);
+  int* off_f = static_cast<int*>(workspace);
+  int* off_a = off_f + b->nprob + 1;
+  batch_offsets_kernel<<<dim3(1), dim3(1024), 0, s>>>(probs, b->nprob, off_f, off_a, 0);
+  TKBN_LAUNCH_CHECK("batch_offsets_kernel");
+  if (b->bound_f == 0) return TKBN_OK;
+  const dim3 grd((unsigned)b->bound_f);
+#define TKBN_BATCH_FWD(DD, JJ) batch_fwd_kernel<DD, JJ><<<grd, dim3(256), b->lds_f, s>>>(probs, b->nprob, off_f)
+  if (b->jm == 6) {
+    if (b->ndim == 1) TKBN_BATCH_FWD(1, 6);
+    else if (b->ndim == 2) TKBN_BATCH_FWD(2, 6);
+    else TKBN_BATCH_FWD(3, 6);
+  } else {
+    if (b->ndim == 1) TKBN_BATCH_FWD(1, 16);
+    else if (b->ndim == 2) TKBN_BATCH_FWD(2, 16);
+    else TKBN_BATCH_FWD(3, 16);
+  }
+#undef TKBN_BATCH_FWD
+  TKBN_LAUNCH_CHECK("batch_fwd_kernel");
+  note_kernel("batch_fwd_kernel");
+  return TKBN_OK;
+}
+
+int tkbn_batch_interp_adj(const tkbn_batch* b, const void* table, void* workspace, int64_t workspace_bytes,
+                          void* stream) {
+  using namespace tkbn;
+  TKBN_REQUIRE(b != nullptr && table != nullptr && workspace != nullptr, TKBN_ERR_INVALID, "tkbn_batch_interp_adj: NULL argument");
+  TKBN_REQUIRE(workspace_bytes >= tkbn_batch_workspace_bytes(b), TKBN_ERR_WORKSPACE, "batch workspace too small");
+  int st = batch_plans_current(b);
+  if (st != TKBN_OK) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const auto* probs = static_cast<const BatchProb*>(table);
+  int* off_f = static_cast<int*>(workspace);
+  int* off_a = off_f + b->nprob + 1;
+  batch_offsets_kernel<<<dim3(1), dim3(1024), 0, s>>>(probs, b->nprob, off_f, off_a, 1);
+  TKBN_LAUNCH_CHECK("batch_offsets_kernel");
+  if (!b->det && b->zblocks > 0) {
+    if (b->ndim == 1) batch_zero_hot_kernel<1><<<dim3((unsigned)b->zblocks), dim3(256), 0, s>>>(probs, b->nprob);
+    else if (b->ndim == 2) batch_zero_hot_kernel<2><<<dim3((unsigned)b->zblocks), dim3(256), 0, s>>>(probs, b->nprob);
+    else batch_zero_hot_kernel<3><<<dim3((unsigned)b->zblocks), dim3(256), 0, s>>>(probs, b->nprob);
+    TKBN_LAUNCH_CHECK("batch_zero_hot_kernel");
+  }
+  batch_maxabs_kernel<<<dim3((unsigned)b->maxabs_blocks, (unsigned)b->nprob), dim3(256), 0, s>>>(probs);
+  TKBN_LAUNCH_CHECK("batch_maxabs_kernel");
+  if (b->bound_a == 0) return TKBN_OK;
+  const dim3 grd((unsigned)b->bound_a);
+#define TKBN_BATCH_ADJ(FX, DD, JJ, FB) \
+  batch_spread_kernel<FX, DD, JJ><<<grd, dim3(256), b->lds_a, s>>>(probs, b->nprob, off_a, FB)
+#define TKBN_BATCH_ADJ_PAIR(DD, JJ)      \
+  do {                                   \
+    TKBN_BATCH_ADJ(true, DD, JJ, 0);     \
+    TKBN_BATCH_ADJ(false, DD, JJ, 1);    \
+  } while (0)
+  if (b->jm == 6) {
+    if (b->ndim == 1) TKBN_BATCH_ADJ_PAIR(1, 6);
+    else if (b->ndim == 2) TKBN_BATCH_ADJ_PAIR(2, 6);
+    else TKBN_BATCH_ADJ_PAIR(3, 6);
+  } else {
+    if (b->ndim == 1) TKBN_BATCH_ADJ_PAIR(1, 16);
+    else if (b->ndim == 2) TKBN_BATCH_ADJ_PAIR(2, 16);
+    else TKBN_BATCH_ADJ_PAIR(3, 16);
+  }
+#undef TKBN_BATCH_ADJ_PAIR
+#undef TKBN_BATCH_ADJ
+  TKBN_LAUNCH_CHECK("batch_spread_kernel");
+  note_kernel(b->det ? "batch_spread_kernel<fixed,deterministic>" : "batch_spread_kernel<fixed>");
+  return TKBN_OK;
+}
+
+}  // extern "C"
+
+namespace tkbn {
 }  // namespace tkbn
 
 // ---- adjoint without device allocation -------------------------------------------
