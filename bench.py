@@ -258,9 +258,22 @@ def bench_batched(dev, iters: int) -> dict:
 
     t_loop = timed(loop, iters)
     ref = loop()
+    # separate launches (one per problem and stage, on 4 side streams) in a graph
+    sep = tkbn.BatchedKbNufft(sizes, ktrajs, coils=coils, device=dev, fused=False, autotune=False)
+    for p, x in enumerate(images):
+        sep.images[p].copy_(x)
+    sep.capture()
+    t_sep_graph = timed(sep.replay, iters)
+    del sep
+    # one launch per stage for all problems, autotuned tiles
     bt = tkbn.BatchedKbNufft(sizes, ktrajs, coils=coils, device=dev)
     for p, x in enumerate(images):
         bt.images[p].copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bt.pair()  # autotune + plans + batch table
+    torch.cuda.synchronize()
+    t_setup = (time.perf_counter() - t0) * 1e3
     t_eager = timed(bt.pair, iters)
     bt.capture()
     t_graph = timed(bt.replay, iters)
@@ -280,9 +293,14 @@ def bench_batched(dev, iters: int) -> dict:
         "(r = pi t^1.5, 16 turns), fwd + adj of every problem per step",
         "samples": int(sum(k.shape[1] for k in ktrajs)),
         "plain_loop_ms": t_loop,
+        "separate_launches_graph_ms": t_sep_graph,
+        "fused": bt.fused,
+        "fused_setup_ms": t_setup,
         "batched_eager_ms": t_eager,
         "batched_graph_ms": t_graph,
         "speedup_graph_vs_loop": t_loop / t_graph,
+        "autotune": {f"{k[0]}x{k[1]}": {d: {"tile": list(v[0][: len(k)]), "work_target": v[1]} for d, v in c.items()}
+                     for k, c in bt.tuning.items()},
         "max_rel_diff_vs_loop": diff,
         "fp64_oracle_rel_err": fp64,
     }
@@ -591,8 +609,9 @@ def main() -> int:
     ap.add_argument("--stage-iters", type=int, default=20)
     ap.add_argument("--form-iters", type=int, default=10, help="iterations of the batched / gradient forms")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
-    ap.add_argument("--workload", default="sense2d", choices=["sense2d", "kooshball3d", "copy512", "spmat"],
+    ap.add_argument("--workload", default="sense2d", choices=["sense2d", "kooshball3d", "copy512", "spmat", "batched"],
                     help="sense2d: the headline SENSE pair (default); kooshball3d: the 3-D deterministic case; "
+                    "batched: the 64 uneven spiral problems only (fused batch, separate launches, plain loop); "
                     "copy512: a plain 512 MiB read + 512 MiB write kernel (calibrates rocprofv3 FETCH/WRITE_SIZE); "
                     "spmat: the SENSE batch with sparse-matrix interpolation only")
     args = ap.parse_args()
@@ -610,6 +629,9 @@ def main() -> int:
     torch.cuda.set_device(dev)
     if args.workload == "kooshball3d":
         print(json.dumps(bench_kooshball(dev, max(1, args.steps // 10), args.warmup)), flush=True)
+        return 0
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(dev, args.form_iters)), flush=True)
         return 0
     if args.workload == "copy512":
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)  # 512 MiB

@@ -282,7 +282,7 @@ def test_c5_reduced_kooshball_matches_oracle(c5_small):
         ref = tp.kb_table_nufft(plan, x, ktraj, smaps)
         assert rel_l2(kd, ref) < C64_TOL
         img = ad(ref.to(DEV), kt, smaps=sm)
-        assert _last_kernel() == "spread_tiles_kernel<fixed,deterministic>"
+        assert _last_kernel() == "spread_reg3_kernel<deterministic>"  # the 3-D register-tile spread
         img2 = ad(ref.to(DEV), kt, smaps=sm)
         assert torch.equal(img, img2)
         err = rel_l2(img, tp.kb_table_nufft_adjoint(plan, ref, ktraj, smaps))

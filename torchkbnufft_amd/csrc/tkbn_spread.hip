@@ -56,6 +56,9 @@ struct TileGeom {
   int64_t nkeys;           // multi-coil plan: (tile, footprint column) keys
   int64_t o_hot, o_cold;   // multi-coil plan: scans of split / unsplit tiles
   int64_t o_rec_bytes;     // multi-coil plan: byte offset of the entry records
+  int64_t o_meta, o_tmp, o_sort, o_scratch;  // 3-D register-tile plan: counts, scan scratch, sort
+  int64_t pool_off_bytes, pool_slots;        // 3-D register-tile plan: piece partials
+  int nchunk;                                // 3-D register-tile plan: chunks of coil rows
 };
 
 // dynamic LDS per group; with the ~350 bytes of static LDS the total stays under 64 KiB
@@ -1944,6 +1947,531 @@ static int spread_sep_launch(const tkbn_interp_desc* d, const void* kdata, const
   return TKBN_OK;
 }
 
+// ---------------------------------------------------------------------------
+// 3-D register-tile spread (complex64, J <= 6, at least kR3MinRows coil rows, one
+// trajectory for all rows, grids of at least tile + J - 1 cells per dimension)
+//   One group (8 waves) per (8 x 8 x 16 tile, piece, chunk of 8 coil rows): wave w owns
+//   the tile's plane x = w; lane (y, row) owns the 16 cells along z of column (w, y) of
+//   its coil row as float2 registers.  Every cell has one writer, so no atomics.
+//   Entries (sample, tile) are staged 64 at a time: each sample's three rows of tap
+//   weights (conjugated), the footprint's offsets inside the tile, and per (entry, y,
+//   row) the sample's data times its y weight.  Each wave then walks the staged entries
+//   that reach its plane, grouped by z offset so that every accumulator index is a
+//   compile-time constant.
+//   A tile with more than P entries is cut into pieces of P entries.  A piece stores its
+//   partial tile into a pool slot and reg3_combine_kernel adds a tile's pieces in piece
+//   order.  The plan raises P until the pieces fit the pool.
+//   Deterministic mode sorts every tile's entries by sample index when the plan is
+//   built, so the pieces, the batches and every float sum have a fixed order: the result
+//   is bitwise repeatable without fixed point.
+// ---------------------------------------------------------------------------
+constexpr int kR3T0 = 8, kR3T1 = 8, kR3T2 = 16;
+constexpr int kR3Rows = 8;                  // coil rows per work item (lanes = y x row)
+constexpr int kR3Threads = 64 * kR3T0;      // one wave per x plane
+constexpr int kR3Batch = 64;                // staged entries
+constexpr int kR3Cells = kR3T0 * kR3T1 * kR3T2;
+constexpr int kR3Target = 1024;             // entries per piece before the pool fit
+constexpr int kR3SortChunk = 2048;          // entries sorted in LDS at once
+constexpr int64_t kR3PoolCap = 128ll << 20;  // bytes of piece partials at most
+constexpr int kR3MinRows = 4;
+constexpr int kR3MinZ = -5;                 // first z offset of a footprint in a tile (J <= 6)
+constexpr int64_t kR3SlotFloats2 = (int64_t)kR3Cells * kR3Rows;
+// plan meta ints: piece size P, hot pieces, hot tiles, sort chunks, merge passes, entries
+enum { kR3MetaP = 0, kR3MetaHotPieces, kR3MetaHotTiles, kR3MetaSort, kR3MetaPasses, kR3MetaEntries, kR3MetaInts = 8 };
+
+static bool reg3_eligible(const tkbn_interp_desc* d) {
+  if (d->dtype != TKBN_F32 || d->ndim != 3 || d->omega_batched) return false;
+  const int T[3] = {kR3T0, kR3T1, kR3T2};
+  for (int i = 0; i < 3; ++i)
+    if (d->numpoints[i] > 6 || d->grid_size[i] < T[i] + d->numpoints[i] - 1) return false;
+  return d->batch * d->coils >= kR3MinRows;
+}
+
+static TileGeom make_reg3_geom(const tkbn_interp_desc* d, bool det) {
+  TileGeom tg{};
+  const int TT[3] = {kR3T0, kR3T1, kR3T2};
+  for (int i = 0; i < 3; ++i) {
+    tg.K[i] = (int)d->grid_size[i];
+    tg.J[i] = (int)d->numpoints[i];
+    tg.T[i] = TT[i];
+    tg.nb[i] = (int)ceil_div(d->grid_size[i], TT[i]);
+    tg.lstride[i] = 1;
+  }
+  tg.bstride[2] = 1;
+  tg.bstride[1] = tg.nb[2];
+  tg.bstride[0] = tg.nb[1] * tg.nb[2];
+  tg.NB = tg.nb[0] * tg.nb[1] * tg.nb[2];
+  tg.tcells = kR3Cells;
+  tg.own_only = 0;
+  tg.CC = kR3Rows;
+  tg.target = kR3Target;
+  // tiles a sample reaches per dimension: 2, or 3 when the last tile is shorter than J - 1
+  int64_t per_sample = 1;
+  for (int i = 0; i < 3; ++i) {
+    int mt = 1;
+    if (tg.nb[i] > 1) {
+      const int last = tg.K[i] - (tg.nb[i] - 1) * tg.T[i];
+      mt = last >= tg.J[i] - 1 ? 2 : 3;
+    }
+    per_sample *= mt;
+  }
+  tg.maxent = per_sample * d->klength;
+  tg.maxwork = tg.NB + tg.maxent / tg.target + 1;
+  int64_t o = 0;
+  tg.o_key = o;
+  tg.o_perm = o; o += tg.maxent;
+  tg.o_count = o; o += tg.NB;
+  tg.o_start = o; o += tg.NB + 1;
+  tg.o_cursor = o; o += tg.NB;
+  tg.o_wstart = o; o += tg.NB + 1;
+  tg.o_work = o; o += tg.maxwork;
+  tg.o_nwork = o; o += 1;
+  tg.o_hot = o; o += tg.NB;
+  tg.o_meta = o; o += kR3MetaInts;
+  tg.o_tmp = o; o += 3 * (tg.NB + 1);
+  const int64_t nsort = det ? tg.NB + tg.maxent / kR3SortChunk + 1 : 0;
+  tg.o_sort = o; o += 2 * nsort;
+  tg.o_scratch = o; o += det ? tg.maxent : 0;
+  o = (o + 63) & ~63LL;  // the pool starts 256-byte aligned
+  tg.plan_ints = o;
+  tg.nchunk = (int)ceil_div(d->batch * d->coils, kR3Rows);
+  // a tile of V > P entries gives ceil(V / P) < 2 V / P pieces
+  const int64_t bound = (2 * tg.maxent / tg.target + 1) * tg.nchunk;
+  const int64_t cap = kR3PoolCap / (kR3SlotFloats2 * 8);
+  tg.pool_slots = std::max<int64_t>(tg.nchunk, std::min<int64_t>(bound, cap));
+  tg.pool_off_bytes = o * 4;
+  return tg;
+}
+
+static int64_t reg3_workspace_bytes(const TileGeom& tg) {
+  return tg.pool_off_bytes + tg.pool_slots * kR3SlotFloats2 * 8;
+}
+
+// block-wide sum or max of one int per thread (every thread gets the result)
+__device__ __forceinline__ int block_reduce_int(int v, bool is_max, int* sb) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const int u = __shfl_down(v, o, 64);
+    v = is_max ? max(v, u) : v + u;
+  }
+  __syncthreads();  // sb may still be read by an earlier reduction
+  if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = sb[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = is_max ? max(r, sb[w]) : r + sb[w];
+  return r;
+}
+
+// Tile starts, the piece size, the work list (hot tiles' pieces first, then one item per
+// other tile, empty tiles included: they store zeros), the hot tile list and, in
+// deterministic mode, the list of sort chunks and the number of merge passes.
+__global__ void __launch_bounds__(1024) reg3_plan_scan_kernel(const TileGeom tg, int* __restrict__ plan,
+                                                             const CacheHdr* cache, int det) {
+  if (cache_hit(cache)) return;  // cached plan reused
+  __shared__ int sbuf[1024];
+  const int NB = tg.NB;
+  const int* count = plan + tg.o_count;
+  int* start = plan + tg.o_start;
+  int* cursor = plan + tg.o_cursor;
+  int* wstart = plan + tg.o_wstart;
+  int* work = plan + tg.o_work;
+  int* hot = plan + tg.o_hot;
+  int* meta = plan + tg.o_meta;
+  int* ta = plan + tg.o_tmp;
+  int* tb = ta + (NB + 1);
+  int* tc = tb + (NB + 1);
+  block_exclusive_scan(count, start, NB, sbuf);
+  int vmax = 0;
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    cursor[t] = start[t];
+    vmax = max(vmax, count[t]);
+  }
+  vmax = block_reduce_int(vmax, true, sbuf);
+  // smallest P = target * 2^k whose hot pieces fit the pool (P >= vmax: no hot tiles)
+  int P = tg.target;
+  for (;;) {
+    int pieces = 0;
+    for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+      const int v = count[t];
+      if (v > P) pieces += (v + P - 1) / P;
+    }
+    pieces = block_reduce_int(pieces, false, sbuf);
+    if ((int64_t)pieces * tg.nchunk <= tg.pool_slots || P >= vmax) break;
+    P *= 2;
+  }
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int v = count[t];
+    ta[t] = v > P ? (v + P - 1) / P : 0;
+  }
+  __syncthreads();
+  block_exclusive_scan(ta, tb, NB, sbuf);  // hot pieces before tile t
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) ta[t] = count[t] > P ? 0 : 1;
+  __syncthreads();
+  block_exclusive_scan(ta, tc, NB, sbuf);  // other tiles before tile t
+  const int nhotp = tb[NB];
+  const int cap = (int)tg.maxwork;
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+    const int v = count[t];
+    const bool h = v > P;
+    const int w0 = h ? tb[t] : nhotp + tc[t];
+    const int n = h ? (v + P - 1) / P : 1;
+    wstart[t] = w0;
+    for (int w = w0; w < min(w0 + n, cap); ++w) work[w] = t;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NB; t += blockDim.x) ta[t] = count[t] > P ? 1 : 0;
+  __syncthreads();
+  block_exclusive_scan(ta, tc, NB, sbuf);  // hot tiles before tile t
+  for (int t = threadIdx.x; t < NB; t += blockDim.x)
+    if (count[t] > P) hot[tc[t]] = t;
+  int nsort = 0;
+  if (det) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+      const int v = count[t];
+      ta[t] = v > 1 ? (v + kR3SortChunk - 1) / kR3SortChunk : 0;
+    }
+    __syncthreads();
+    block_exclusive_scan(ta, tb, NB, sbuf);  // sort chunks before tile t
+    int* sortl = plan + tg.o_sort;
+    for (int t = threadIdx.x; t < NB; t += blockDim.x) {
+      const int v = count[t];
+      if (v <= 1) continue;
+      for (int c = 0; c * kR3SortChunk < v; ++c) {
+        sortl[2 * (tb[t] + c)] = start[t] + c * kR3SortChunk;
+        sortl[2 * (tb[t] + c) + 1] = min(kR3SortChunk, v - c * kR3SortChunk);
+      }
+    }
+    nsort = tb[NB];
+  }
+  if (threadIdx.x == 0) {
+    int passes = 0;
+    if (det)
+      while (((int64_t)kR3SortChunk << passes) < vmax) ++passes;
+    meta[kR3MetaP] = P;
+    meta[kR3MetaHotPieces] = nhotp;
+    meta[kR3MetaHotTiles] = tc[NB];
+    meta[kR3MetaSort] = nsort;
+    meta[kR3MetaPasses] = passes;
+    meta[kR3MetaEntries] = start[NB];
+    wstart[NB] = nhotp + (NB - tc[NB]);
+    plan[tg.o_nwork] = min(nhotp + (NB - tc[NB]), cap);
+  }
+}
+
+// Sorts each chunk of at most kR3SortChunk entries of a tile by sample index (bitonic,
+// in LDS; padding sorts last).
+__global__ void __launch_bounds__(1024) reg3_sort_chunks_kernel(const TileGeom tg, int* __restrict__ plan,
+                                                               const CacheHdr* cache) {
+  if (cache_hit(cache)) return;
+  if ((int)blockIdx.x >= plan[tg.o_meta + kR3MetaSort]) return;
+  __shared__ int buf[kR3SortChunk];
+  const int pos = plan[tg.o_sort + 2 * blockIdx.x], len = plan[tg.o_sort + 2 * blockIdx.x + 1];
+  int* perm = plan + tg.o_perm;
+  for (int i = threadIdx.x; i < kR3SortChunk; i += blockDim.x) buf[i] = i < len ? perm[pos + i] : 0x7fffffff;
+  __syncthreads();
+  const int tid = threadIdx.x;  // 1024 threads: one compare-exchange pair each
+  for (int k = 2; k <= kR3SortChunk; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int i = 2 * tid - (tid & (j - 1));
+      const int p = i + j;
+      const bool up = (i & k) == 0;
+      const int a = buf[i], b = buf[p];
+      if ((a > b) == up) {
+        buf[i] = b;
+        buf[p] = a;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < len; i += blockDim.x) perm[pos + i] = buf[i];
+}
+
+// elements of the sorted run a[lo .. hi) smaller than key
+__device__ __forceinline__ int count_less(const int* a, int lo, int hi, int key) {
+  const int base = lo;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo - base;
+}
+
+// Merge pass `pass`: inside every tile longer than R = kR3SortChunk << pass, sorted runs
+// [2kR, (2k+1)R) and [(2k+1)R, (2k+2)R) (tile-relative) become one sorted run.  Keys
+// are distinct, so each element's place is its run offset plus its rank in the partner
+// run.  Passes alternate between the permutation and the scratch array; passes past
+// the plan's count do nothing.
+__global__ void __launch_bounds__(256) reg3_merge_kernel(const TileGeom tg, int* __restrict__ plan,
+                                                        const CacheHdr* cache, int pass) {
+  if (cache_hit(cache)) return;
+  const int* meta = plan + tg.o_meta;
+  if (pass >= meta[kR3MetaPasses]) return;
+  const int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p64 >= meta[kR3MetaEntries]) return;
+  const int p = (int)p64;
+  const int* src = plan + ((pass & 1) ? tg.o_scratch : tg.o_perm);
+  int* dst = plan + ((pass & 1) ? tg.o_perm : tg.o_scratch);
+  const int* start = plan + tg.o_start;
+  int lo = 0, hi = tg.NB;  // last tile t with start[t] <= p
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= p) lo = mid;
+    else hi = mid;
+  }
+  const int s0 = start[lo], V = start[lo + 1] - s0, o = p - s0;
+  const int R = kR3SortChunk << pass;
+  const int key = src[p];
+  if (V <= R) {
+    dst[p] = key;
+    return;
+  }
+  const int r = o / R, base = (r & ~1) * R;
+  int no;
+  if ((r & 1) == 0) no = o + count_less(src + s0, min(base + R, V), min(base + 2 * R, V), key);
+  else no = base + (o - base - R) + count_less(src + s0, base, base + R, key);
+  dst[s0 + no] = key;
+}
+
+// acc[C + j] += u * w[j] for the taps that land in the tile (C = the footprint's z offset)
+template <int C>
+__device__ __forceinline__ void r3_add_z(float2 (&acc)[kR3T2], const float2 u, const float2* w) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    if (C + j >= 0 && C + j < kR3T2) {
+      const float2 t = w[j];
+      acc[C + j].x = fmaf(u.x, t.x, acc[C + j].x);
+      acc[C + j].x = fmaf(-u.y, t.y, acc[C + j].x);
+      acc[C + j].y = fmaf(u.x, t.y, acc[C + j].y);
+      acc[C + j].y = fmaf(u.y, t.x, acc[C + j].y);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kR3Threads)
+    spread_reg3_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ kdata,
+                       const float* __restrict__ omega, const int* __restrict__ plan, float2* __restrict__ grid,
+                       float2* __restrict__ pool) {
+  __shared__ __attribute__((aligned(16))) float2 W0[kR3Batch][6];
+  __shared__ __attribute__((aligned(16))) float2 W1[kR3Batch][6];
+  __shared__ __attribute__((aligned(16))) float2 W2[kR3Batch][6];
+  __shared__ float2 S[kR3Batch][kR3Rows];
+  __shared__ float2 Vy[kR3Batch][64];
+  __shared__ int R0[kR3Batch], R1[kR3Batch], R2[kR3Batch];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int nchunk = tg.nchunk;
+  const int64_t wi = blockIdx.x / nchunk;
+  const int chunk = (int)(blockIdx.x - wi * nchunk);
+  if (wi >= plan[tg.o_nwork]) return;
+  const int* meta = plan + tg.o_meta;
+  const int P = meta[kR3MetaP];
+  const int t = plan[tg.o_work + wi];
+  const int piece = (int)(wi - plan[tg.o_wstart + t]);
+  const int s0 = plan[tg.o_start + t];
+  const int V = plan[tg.o_start + t + 1] - s0;
+  const bool split = V > P;
+  const int i0 = split ? piece * P : 0;
+  const int i1 = split ? min(i0 + P, V) : V;
+  const int* perm = plan + ((meta[kR3MetaPasses] & 1) ? tg.o_scratch : tg.o_perm) + s0;
+  const int64_t nrows = g.B * g.C;
+  const int64_t rbase = (int64_t)chunk * kR3Rows;
+  const int ncc = (int)(rbase + kR3Rows < nrows ? kR3Rows : nrows - rbase);
+  int org[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) org[d] = ((t / tg.bstride[d]) % tg.nb[d]) * tg.T[d];
+
+  float2 acc[kR3T2];
+#pragma unroll
+  for (int z = 0; z < kR3T2; ++z) acc[z] = make_float2(0.f, 0.f);
+
+  for (int base = i0; base < i1; base += kR3Batch) {
+    const int nb = min(kR3Batch, i1 - base);
+    {
+      // stage: thread (part, e) computes the weight row of dimension `part` (parts 0..2)
+      // and the data of coil row `part` times the sample's phase, of entry e
+      const int e = threadIdx.x & 63, part = threadIdx.x >> 6;
+      const bool live = e < nb;
+      const int m = live ? perm[base + e] : 0;
+      if (part < 3) {
+        float2(*Wd)[6] = part == 0 ? W0 : (part == 1 ? W1 : W2);
+        int* Rd = part == 0 ? R0 : (part == 1 ? R1 : R2);
+        float2 w[6];
+        int mdx[6];
+        int r = -100;  // no tap of a dead lane lands in the tile
+        if (live) {
+          sample_taps_md<float, 6>(g, part, omega[part * g.M + m], w, mdx);
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            const int l = mdx[j] - org[part];
+            if (mdx[j] >= 0 && l >= 0 && l < tg.T[part]) r = l - j;  // one run per tile
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < 6; ++j) w[j] = make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < 6; ++j) Wd[e][j] = make_float2(w[j].x, -w[j].y);  // zero past J
+        Rd[e] = r;
+      }
+      float2 sv = make_float2(0.f, 0.f);
+      if (live && part < ncc) {
+        const float theta = sample_theta<float, 3>(g, omega, m);
+        const float2 ph = make_float2(cosf(theta), sinf(theta));
+        sv = cmul_conj(kdata[(rbase + part) * g.M + m], ph);
+      }
+      S[e][part] = sv;
+    }
+    __syncthreads();
+    // data times the y weight for every (entry, y, row)
+#pragma unroll
+    for (int k = 0; k < (kR3Batch * 64) / kR3Threads; ++k) {
+      const int idx = (int)threadIdx.x + k * kR3Threads;
+      const int e = idx >> 6, l = idx & 63, y = l >> 3, q = l & 7;
+      const int j1 = y - R1[e];
+      float2 v = make_float2(0.f, 0.f);
+      if (j1 >= 0 && j1 < 6) v = cmul(S[e][q], W1[e][j1]);
+      Vy[e][l] = v;
+    }
+    __syncthreads();
+    // walk: entries that reach this wave's plane, by z offset
+    const int r0l = R0[lane], r2l = R2[lane];
+    const int jxl = wave - r0l;
+    const unsigned long long hits = __ballot(jxl >= 0 && jxl < g.J[0]);
+#define TKBN_R3_GROUP(C)                                                     \
+  {                                                                          \
+    unsigned long long mk = __ballot(r2l == (C)) & hits;                     \
+    while (mk) {                                                             \
+      const int e = __ffsll(mk) - 1;                                         \
+      mk &= mk - 1;                                                          \
+      const int jx = wave - __builtin_amdgcn_readlane(r0l, e);               \
+      const float2 u = cmul(Vy[e][lane], W0[e][jx]);                         \
+      r3_add_z<C>(acc, u, W2[e]);                                            \
+    }                                                                        \
+  }
+    TKBN_R3_GROUP(-5) TKBN_R3_GROUP(-4) TKBN_R3_GROUP(-3) TKBN_R3_GROUP(-2) TKBN_R3_GROUP(-1)
+    TKBN_R3_GROUP(0) TKBN_R3_GROUP(1) TKBN_R3_GROUP(2) TKBN_R3_GROUP(3) TKBN_R3_GROUP(4)
+    TKBN_R3_GROUP(5) TKBN_R3_GROUP(6) TKBN_R3_GROUP(7) TKBN_R3_GROUP(8) TKBN_R3_GROUP(9)
+    TKBN_R3_GROUP(10) TKBN_R3_GROUP(11) TKBN_R3_GROUP(12) TKBN_R3_GROUP(13) TKBN_R3_GROUP(14)
+    TKBN_R3_GROUP(15)
+#undef TKBN_R3_GROUP
+    __syncthreads();
+  }
+
+  const int y = lane >> 3, q = lane & 7;
+  if (split) {
+    // partial tile of this piece: slot (wi, chunk), element ((q * 8 + x) * 8 + y) * 16 + z
+    float2* dst = pool + ((int64_t)wi * nchunk + chunk) * kR3SlotFloats2 + ((q * kR3T0 + wave) * kR3T1 + y) * kR3T2;
+#pragma unroll
+    for (int z = 0; z < kR3T2; ++z) dst[z] = acc[z];
+    return;
+  }
+  const int k0 = org[0] + wave, k1 = org[1] + y;
+  if (q < ncc && k0 < g.K[0] && k1 < g.K[1]) {
+    float2* dst = grid + (rbase + q) * g.G + (int64_t)k0 * g.stride[0] + (int64_t)k1 * g.stride[1] + org[2];
+#pragma unroll
+    for (int z = 0; z < kR3T2; ++z)
+      if (org[2] + z < g.K[2]) dst[z] = acc[z];
+  }
+}
+
+// Hot tile h, coil chunk blockIdx.y: the sum of its pieces' partial tiles, in piece order.
+__global__ void __launch_bounds__(256) reg3_combine_kernel(const Geom<float> g, const TileGeom tg,
+                                                          const int* __restrict__ plan,
+                                                          const float2* __restrict__ pool, float2* __restrict__ grid) {
+  const int* meta = plan + tg.o_meta;
+  const int h = blockIdx.x;
+  if (h >= meta[kR3MetaHotTiles]) return;
+  const int nchunk = tg.nchunk, chunk = blockIdx.y;
+  const int P = meta[kR3MetaP];
+  const int t = plan[tg.o_hot + h];
+  const int w0 = plan[tg.o_wstart + t];
+  const int V = plan[tg.o_start + t + 1] - plan[tg.o_start + t];
+  const int np = (V + P - 1) / P;
+  const int64_t nrows = g.B * g.C;
+  const int64_t rbase = (int64_t)chunk * kR3Rows;
+  const int ncc = (int)(rbase + kR3Rows < nrows ? kR3Rows : nrows - rbase);
+  int org[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) org[d] = ((t / tg.bstride[d]) % tg.nb[d]) * tg.T[d];
+  for (int idx = threadIdx.x; idx < kR3SlotFloats2; idx += blockDim.x) {
+    const int z = idx & (kR3T2 - 1), y = (idx / kR3T2) & (kR3T1 - 1), x = (idx / (kR3T2 * kR3T1)) & (kR3T0 - 1);
+    const int q = idx / kR3Cells;
+    const int k0 = org[0] + x, k1 = org[1] + y, k2 = org[2] + z;
+    if (q >= ncc || k0 >= g.K[0] || k1 >= g.K[1] || k2 >= g.K[2]) continue;
+    float2 s = pool[((int64_t)w0 * nchunk + chunk) * kR3SlotFloats2 + idx];
+    for (int k = 1; k < np; ++k) {
+      const float2 v = pool[((int64_t)(w0 + k) * nchunk + chunk) * kR3SlotFloats2 + idx];
+      s.x = s.x + v.x;
+      s.y = s.y + v.y;
+    }
+    grid[(rbase + q) * g.G + (int64_t)k0 * g.stride[0] + (int64_t)k1 * g.stride[1] + k2] = s;
+  }
+}
+
+constexpr int kPlanReg3 = 8;  // + 1 for deterministic mode (sorted entries)
+
+static int reg3_spread_launch(const tkbn_interp_desc* d, bool det, const void* kdata, const void* omega, void* ws,
+                              void* grid, hipStream_t s) {
+  const Geom<float> g = make_geom<float>(d);
+  const TileGeom tg = make_reg3_geom(d, det);
+  TKBN_REQUIRE(tg.maxent + tg.maxwork < (1LL << 31) && tg.plan_ints < (1LL << 31), TKBN_ERR_UNSUPPORTED,
+               "3-D spread plan too large (%lld entries)", (long long)tg.maxent);
+  const int kind = kPlanReg3 + (det ? 1 : 0);
+  const bool reuse = plan_reused(d);
+  CacheHdr* cache = nullptr;
+  int st = TKBN_OK;
+  int* plan = reinterpret_cast<int*>(static_cast<char*>(ws) + plan_cache_bytes(d));
+  float2* pool = reinterpret_cast<float2*>(reinterpret_cast<char*>(plan) + tg.pool_off_bytes);
+  if (!reuse) {
+    CacheZero z;  // the tile counts
+    z.p = plan + tg.o_count;
+    z.n = tg.NB;
+    z.stride = tg.plan_ints;
+    z.copies = 1;
+    st = plan_cache_begin(d, kind, tg.T, omega, ws, s, &cache, z);
+    if (st != TKBN_OK) return st;
+  } else {
+    st = plan_registry_require(d, kind, tg.T, ws);
+    if (st != TKBN_OK) return st;
+  }
+  const float* om = static_cast<const float*>(omega);
+  if (cache) {
+    const dim3 gc((unsigned)std::max<int64_t>(1, ceil_div(g.M, 256)));
+    if (g.M > 0) {
+      bin_count_kernel<float, 3><<<gc, dim3(256), 0, s>>>(g, tg, om, plan, cache);
+      TKBN_LAUNCH_CHECK("bin_count_kernel<reg3>");
+    }
+    reg3_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, plan, cache, det ? 1 : 0);
+    TKBN_LAUNCH_CHECK("reg3_plan_scan_kernel");
+    if (g.M > 0) {
+      bin_scatter_kernel<float, 3><<<gc, dim3(256), 0, s>>>(g, tg, om, plan, cache);
+      TKBN_LAUNCH_CHECK("bin_scatter_kernel<reg3>");
+    }
+    if (det && g.M > 0) {
+      const int64_t nsort = tg.NB + tg.maxent / kR3SortChunk + 1;
+      reg3_sort_chunks_kernel<<<dim3((unsigned)nsort), dim3(1024), 0, s>>>(tg, plan, cache);
+      TKBN_LAUNCH_CHECK("reg3_sort_chunks_kernel");
+      const dim3 gm((unsigned)ceil_div(tg.maxent, 256));
+      for (int pass = 0; ((int64_t)kR3SortChunk << pass) < tg.maxent; ++pass) {
+        reg3_merge_kernel<<<gm, dim3(256), 0, s>>>(tg, plan, cache, pass);
+        TKBN_LAUNCH_CHECK("reg3_merge_kernel");
+      }
+    }
+  }
+  profile_before(s);
+  spread_reg3_kernel<<<dim3((unsigned)(tg.maxwork * tg.nchunk)), dim3(kR3Threads), 0, s>>>(
+      g, tg, static_cast<const float2*>(kdata), om, plan, static_cast<float2*>(grid), pool);
+  profile_after(s);
+  TKBN_LAUNCH_CHECK("spread_reg3_kernel");
+  const int64_t hot_bound = std::max<int64_t>(1, std::min<int64_t>(tg.NB, tg.pool_slots / tg.nchunk));
+  reg3_combine_kernel<<<dim3((unsigned)hot_bound, (unsigned)tg.nchunk), dim3(256), 0, s>>>(
+      g, tg, plan, pool, static_cast<float2*>(grid));
+  TKBN_LAUNCH_CHECK("reg3_combine_kernel");
+  note_kernel(det ? "spread_reg3_kernel<deterministic>" : "spread_reg3_kernel");
+  return TKBN_OK;
+}
+
 constexpr int kPlanSpreadTiles = 3;  // + 1 for the deterministic tile geometry
 constexpr int kPlanFwdTiles = 5;
 
@@ -1957,6 +2485,7 @@ static int spread_dispatch(const Geom<T>& g, const TileGeom& tg, bool det, const
 template <typename T>
 int interp_adj_t(const tkbn_interp_desc* d, bool det, const void* kdata, const void* omega, void* grid, void* ws,
                  hipStream_t s) {
+  if (std::is_same<T, float>::value && reg3_eligible(d)) return reg3_spread_launch(d, det, kdata, omega, ws, grid, s);
   if (std::is_same<T, float>::value && !det && coil_variant_eligible(d))
     return spread_sep_launch(d, kdata, omega, ws, grid, s);
   const Geom<T> g = make_geom<T>(d);
@@ -2320,7 +2849,7 @@ int tkbn_batch_create(const tkbn_interp_desc* descs, const tkbn_interp_desc* adj
       st = TKBN_ERR_UNSUPPORTED;
     }
     if (st == TKBN_OK && (gather_coils_eligible(d) || interp_fwd_workspace_bytes(d) <= 0 ||
-                          (!b->det && coil_variant_eligible(da)))) {
+                          (!b->det && coil_variant_eligible(da)) || reg3_eligible(da))) {
       set_error("tkbn_batch_create: problem %d takes the multi-coil or unbinned path; batches run the sample-lane "
                 "kernels only (fewer than 32 coil rows)", p);
       st = TKBN_ERR_UNSUPPORTED;
@@ -2557,6 +3086,10 @@ int64_t tkbn_interp_adj_workspace_bytes(const tkbn_interp_desc* desc, int32_t mo
     SepGeom sg;
     const int64_t nc = plan_cache_bytes(desc) + sep_workspace_bytes(make_sep_spread_geom(desc, &sg), false);
     n = nc > n ? nc : n;
+  }
+  if (reg3_eligible(desc)) {
+    const int64_t nr = plan_cache_bytes(desc) + reg3_workspace_bytes(make_reg3_geom(desc, det));
+    n = nr > n ? nr : n;
   }
   return n;
 }
