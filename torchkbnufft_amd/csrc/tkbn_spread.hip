@@ -57,6 +57,7 @@ struct TileGeom {
   int64_t o_hot, o_cold;   // multi-coil plan: scans of split / unsplit tiles
   int64_t o_rec_bytes;     // multi-coil plan: byte offset of the entry records
   int64_t o_meta, o_tmp, o_sort, o_scratch;  // 3-D register-tile plan: counts, scan scratch, sort
+  int64_t o_sw, o_sb;                        // 3-D register-tile plan: per-sample tap weights, bases
   int64_t pool_off_bytes, pool_slots;        // 3-D register-tile plan: piece partials
   int nchunk;                                // 3-D register-tile plan: chunks of coil rows
 };
@@ -183,19 +184,24 @@ __device__ __forceinline__ void for_each_tile(const TileGeom& tg, int tl[3][3], 
       }
 }
 
+// Grid-stride over the samples (whole blocks step together, so the wave-aggregated
+// atomics see full waves); the grid is capped (bin_grid) so that the launch costs
+// little when a cached plan makes it exit at once.
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
     bin_count_kernel(const Geom<T> g, const TileGeom tg, const T* __restrict__ omega, int* __restrict__ ws,
                      const CacheHdr* cache) {
   if (cache_hit(cache)) return;  // cached plan reused
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = m < g.M;
   const int64_t b = blockIdx.z;
   const T* om = omega + (g.batched ? b * D * g.M : 0);
   int* count = ws + b * tg.plan_ints + tg.o_count;
-  int tl[3][3], ntl[3];
-  sample_tiles<T, D>(g, tg, om, m, live, tl, ntl);
-  for_each_tile<D>(tg, tl, ntl, live, [&](int key, bool valid) { wave_aggregated_add(count, key, valid); });
+  for (int64_t m0 = (int64_t)blockIdx.x * blockDim.x; m0 < g.M; m0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = m0 + threadIdx.x;
+    const bool live = m < g.M;
+    int tl[3][3], ntl[3];
+    sample_tiles<T, D>(g, tg, om, m, live, tl, ntl);
+    for_each_tile<D>(tg, tl, ntl, live, [&](int key, bool valid) { wave_aggregated_add(count, key, valid); });
+  }
 }
 
 template <typename T, int D>
@@ -203,18 +209,23 @@ __global__ void __launch_bounds__(256)
     bin_scatter_kernel(const Geom<T> g, const TileGeom tg, const T* __restrict__ omega, int* __restrict__ ws,
                        const CacheHdr* cache) {
   if (cache_hit(cache)) return;  // cached plan reused
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = m < g.M;
   const int64_t b = blockIdx.z;
   const T* om = omega + (g.batched ? b * D * g.M : 0);
   int* plan = ws + b * tg.plan_ints;
-  int tl[3][3], ntl[3];
-  sample_tiles<T, D>(g, tg, om, m, live, tl, ntl);
-  for_each_tile<D>(tg, tl, ntl, live, [&](int key, bool valid) {
-    const int pos = wave_aggregated_inc(plan + tg.o_cursor, key, valid);
-    if (valid) plan[tg.o_perm + pos] = (int)m;
-  });
+  for (int64_t m0 = (int64_t)blockIdx.x * blockDim.x; m0 < g.M; m0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = m0 + threadIdx.x;
+    const bool live = m < g.M;
+    int tl[3][3], ntl[3];
+    sample_tiles<T, D>(g, tg, om, m, live, tl, ntl);
+    for_each_tile<D>(tg, tl, ntl, live, [&](int key, bool valid) {
+      const int pos = wave_aggregated_inc(plan + tg.o_cursor, key, valid);
+      if (valid) plan[tg.o_perm + pos] = (int)m;
+    });
+  }
 }
+
+// blocks of 256 for a grid-stride pass over n items: at most 2048
+inline unsigned bin_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 2048)); }
 
 // Block-wide exclusive scan of n ints (in place into out[0..n]), 1024 threads.
 __device__ void block_exclusive_scan(const int* in, int* out, int n, int* sbuf) {
@@ -958,6 +969,12 @@ __device__ __forceinline__ void sep_table_prep(const Geom<float>& g, const SepGe
 }
 
 // ---- plans -------------------------------------------------------------------------
+__device__ __forceinline__ void sep_gat_record(const Geom<float>& g, const TileGeom& tg, const SepGeom& sg,
+                                               const float* __restrict__ omega, int* __restrict__ ws, int m, int key,
+                                               const int (&l)[2], int pos);
+__device__ __forceinline__ void sep_spr_scatter_item(const Geom<float>& g, const TileGeom& tg, const SepGeom& sg,
+                                                     const float* __restrict__ omega, int* __restrict__ ws, int m);
+
 // Forward: each sample once, in the 16 x 16 tile of its base cell (trajectory order
 // within a tile as far as the building atomics keep it); count[NB] is the table flag.
 __global__ void __launch_bounds__(256)
@@ -965,23 +982,33 @@ __global__ void __launch_bounds__(256)
                          int* __restrict__ ws, const CacheHdr* cache) {
   if (cache_hit(cache)) return;  // cached plan reused
   sep_table_prep(g, sg, ws, ws + tg.o_count + tg.NB);
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = m < g.M;
-  int l[2];
-  const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
-  wave_aggregated_add(ws + tg.o_count, key, live);
+  for (int m0 = blockIdx.x * blockDim.x; m0 < g.M; m0 += gridDim.x * blockDim.x) {
+    const int m = m0 + threadIdx.x;
+    const bool live = m < g.M;
+    int l[2];
+    const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
+    wave_aggregated_add(ws + tg.o_count, key, live);
+  }
 }
 
 __global__ void __launch_bounds__(256)
     sep_gat_scatter_kernel(const Geom<float> g, const TileGeom tg, const SepGeom sg, const float* __restrict__ omega,
                            int* __restrict__ ws, const CacheHdr* cache) {
   if (cache_hit(cache)) return;  // cached plan reused
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = m < g.M;
-  int l[2] = {0, 0};
-  const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
-  const int pos = wave_aggregated_inc(ws + tg.o_cursor, key, live);
-  if (!live) return;
+  for (int m0 = blockIdx.x * blockDim.x; m0 < g.M; m0 += gridDim.x * blockDim.x) {
+    const int m = m0 + threadIdx.x;
+    const bool live = m < g.M;
+    int l[2] = {0, 0};
+    const int key = live ? gat_base_tile(g, tg, omega, m, l) : 0;
+    const int pos = wave_aggregated_inc(ws + tg.o_cursor, key, live);
+    if (live) sep_gat_record(g, tg, sg, omega, ws, m, key, l, pos);
+  }
+}
+
+// the forward record of sample m (tile `key`, base cell l inside it) at plan position pos
+__device__ __forceinline__ void sep_gat_record(const Geom<float>& g, const TileGeom& tg, const SepGeom& sg,
+                                               const float* __restrict__ omega, int* __restrict__ ws, int m, int key,
+                                               const int (&l)[2], int pos) {
   const bool tab_ok = ws[tg.o_count + tg.NB] == 0;
   const int nm10 = ws[sg.o_nm1], nm11 = ws[sg.o_nm1 + 1];
   const float om0 = omega[m], om1 = omega[g.M + m];
@@ -1024,22 +1051,31 @@ __global__ void __launch_bounds__(256)
                          int* __restrict__ ws, const CacheHdr* cache) {
   if (cache_hit(cache)) return;  // cached plan reused
   sep_table_prep(g, sg, ws, ws + tg.o_count + tg.NB);
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = m < g.M;
-  int tl[2][3], off[2][3], n[2];
-  coil_sample_tiles(g, tg, omega, m, live, tl, off, n);
-  int* count = ws + tg.o_count;
-  coil_for_each_entry(tg, tl, off, n, live, [&](int key, int, int c, bool valid) {
-    const int tkey = valid ? (key - (off[1][c] - kRegMinC0)) / kRegKeys : 0;
-    wave_aggregated_add(count, tkey, valid);
-  });
+  for (int m0 = blockIdx.x * blockDim.x; m0 < g.M; m0 += gridDim.x * blockDim.x) {
+    const int m = m0 + threadIdx.x;
+    const bool live = m < g.M;
+    int tl[2][3], off[2][3], n[2];
+    coil_sample_tiles(g, tg, omega, m, live, tl, off, n);
+    int* count = ws + tg.o_count;
+    coil_for_each_entry(tg, tl, off, n, live, [&](int key, int, int c, bool valid) {
+      const int tkey = valid ? (key - (off[1][c] - kRegMinC0)) / kRegKeys : 0;
+      wave_aggregated_add(count, tkey, valid);
+    });
+  }
 }
 
 __global__ void __launch_bounds__(256)
     sep_spr_scatter_kernel(const Geom<float> g, const TileGeom tg, const SepGeom sg, const float* __restrict__ omega,
                            int* __restrict__ ws, const CacheHdr* cache) {
   if (cache_hit(cache)) return;  // cached plan reused
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int m0 = blockIdx.x * blockDim.x; m0 < g.M; m0 += gridDim.x * blockDim.x)
+    sep_spr_scatter_item(g, tg, sg, omega, ws, m0 + threadIdx.x);
+}
+
+// the adjoint entries of sample m (m >= M: a dead lane that still takes part in the
+// wave-aggregated cursor increments)
+__device__ __forceinline__ void sep_spr_scatter_item(const Geom<float>& g, const TileGeom& tg, const SepGeom& sg,
+                                                     const float* __restrict__ omega, int* __restrict__ ws, int m) {
   const bool live = m < g.M;
   int tl[2][3], off[2][3], n[2];
   coil_sample_tiles(g, tg, omega, m, live, tl, off, n);
@@ -1665,7 +1701,7 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t zdim = g.batched ? g.B : 1;
   if (cache && g.M > 0) {  // no header: the caller reuses the plan
-    dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
+    dim3 gc(bin_grid(g.M), 1, (unsigned)zdim);
     bin_count_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
     TKBN_LAUNCH_CHECK("bin_count_kernel");
   }
@@ -1674,7 +1710,7 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
     TKBN_LAUNCH_CHECK("plan_scan_kernel");
   }
   if (cache && g.M > 0) {
-    dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
+    dim3 gc(bin_grid(g.M), 1, (unsigned)zdim);
     bin_scatter_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
     TKBN_LAUNCH_CHECK("bin_scatter_kernel");
   }
@@ -1868,7 +1904,7 @@ static int gather_sep_launch(const tkbn_interp_desc* d, const void* grid, const 
     z.n = tg.NB + 1;
     int st = plan_cache_begin(d, kPlanSepGather, tg.T, omega, ws, s, &cache, z);
     if (st != TKBN_OK) return st;
-    dim3 gc((unsigned)std::max<int64_t>(1, ceil_div(g.M, 256)), 1, 1);
+    const dim3 gc(bin_grid(g.M));
     sep_gat_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, sg, om, wsi, cache);
     hipError_t e1 = hipGetLastError();
     sep_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache, tg.target, tg.target, 1);
@@ -1915,7 +1951,7 @@ static int spread_sep_launch(const tkbn_interp_desc* d, const void* kdata, const
     z.n = tg.NB + 1;
     int st = plan_cache_begin(d, kPlanSepSpread, tg.T, omega, ws, s, &cache, z);
     if (st != TKBN_OK) return st;
-    dim3 gc((unsigned)std::max<int64_t>(1, ceil_div(g.M, 256)), 1, 1);
+    const dim3 gc(bin_grid(g.M));
     sep_spr_count_kernel<<<gc, dim3(256), 0, s>>>(g, tg, sg, om, wsi, cache);
     hipError_t e1 = hipGetLastError();
     sep_plan_scan_kernel<<<dim3(1), dim3(1024), 0, s>>>(tg, wsi, cache, kRegMaxUnsplit, kRegHotTarget, 0);
@@ -1976,6 +2012,9 @@ constexpr int64_t kR3PoolCap = 128ll << 20;  // bytes of piece partials at most
 constexpr int kR3MinRows = 4;
 constexpr int kR3MinZ = -5;                 // first z offset of a footprint in a tile (J <= 6)
 constexpr int64_t kR3SlotFloats2 = (int64_t)kR3Cells * kR3Rows;
+constexpr int kR3FlushPitch = kR3T2 + 1;  // float2 per transposed column (odd: fewer bank conflicts)
+constexpr int kR3LdsBytes = kR3T0 * 64 * kR3FlushPitch * 8;  // >= the staged batch (46848 bytes)
+static_assert(kR3LdsBytes >= 46848, "flush buffer covers the staging layout");
 // plan meta ints: piece size P, hot pieces, hot tiles, sort chunks, merge passes, entries
 enum { kR3MetaP = 0, kR3MetaHotPieces, kR3MetaHotTiles, kR3MetaSort, kR3MetaPasses, kR3MetaEntries, kR3MetaInts = 8 };
 
@@ -2032,6 +2071,9 @@ static TileGeom make_reg3_geom(const tkbn_interp_desc* d, bool det) {
   const int64_t nsort = det ? tg.NB + tg.maxent / kR3SortChunk + 1 : 0;
   tg.o_sort = o; o += 2 * nsort;
   tg.o_scratch = o; o += det ? tg.maxent : 0;
+  o = (o + 3) & ~3LL;
+  tg.o_sw = o; o += 3 * d->klength * 12;  // float2 [3][M][6]
+  tg.o_sb = o; o += 3 * d->klength;       // int [3][M]
   o = (o + 63) & ~63LL;  // the pool starts 256-byte aligned
   tg.plan_ints = o;
   tg.nchunk = (int)ceil_div(d->batch * d->coils, kR3Rows);
@@ -2163,27 +2205,30 @@ __global__ void __launch_bounds__(1024) reg3_plan_scan_kernel(const TileGeom tg,
 __global__ void __launch_bounds__(1024) reg3_sort_chunks_kernel(const TileGeom tg, int* __restrict__ plan,
                                                                const CacheHdr* cache) {
   if (cache_hit(cache)) return;
-  if ((int)blockIdx.x >= plan[tg.o_meta + kR3MetaSort]) return;
+  const int nsort = plan[tg.o_meta + kR3MetaSort];
   __shared__ int buf[kR3SortChunk];
-  const int pos = plan[tg.o_sort + 2 * blockIdx.x], len = plan[tg.o_sort + 2 * blockIdx.x + 1];
   int* perm = plan + tg.o_perm;
-  for (int i = threadIdx.x; i < kR3SortChunk; i += blockDim.x) buf[i] = i < len ? perm[pos + i] : 0x7fffffff;
-  __syncthreads();
-  const int tid = threadIdx.x;  // 1024 threads: one compare-exchange pair each
-  for (int k = 2; k <= kR3SortChunk; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      const int i = 2 * tid - (tid & (j - 1));
-      const int p = i + j;
-      const bool up = (i & k) == 0;
-      const int a = buf[i], b = buf[p];
-      if ((a > b) == up) {
-        buf[i] = b;
-        buf[p] = a;
+  for (int c = blockIdx.x; c < nsort; c += gridDim.x) {
+    const int pos = plan[tg.o_sort + 2 * c], len = plan[tg.o_sort + 2 * c + 1];
+    for (int i = threadIdx.x; i < kR3SortChunk; i += blockDim.x) buf[i] = i < len ? perm[pos + i] : 0x7fffffff;
+    __syncthreads();
+    const int tid = threadIdx.x;  // 1024 threads: one compare-exchange pair each
+    for (int k = 2; k <= kR3SortChunk; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int i = 2 * tid - (tid & (j - 1));
+        const int p = i + j;
+        const bool up = (i & k) == 0;
+        const int a = buf[i], b = buf[p];
+        if ((a > b) == up) {
+          buf[i] = b;
+          buf[p] = a;
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
+    for (int i = threadIdx.x; i < len; i += blockDim.x) perm[pos + i] = buf[i];
+    __syncthreads();  // buf is refilled by the next chunk
   }
-  for (int i = threadIdx.x; i < len; i += blockDim.x) perm[pos + i] = buf[i];
 }
 
 // elements of the sorted run a[lo .. hi) smaller than key
@@ -2207,30 +2252,32 @@ __global__ void __launch_bounds__(256) reg3_merge_kernel(const TileGeom tg, int*
   if (cache_hit(cache)) return;
   const int* meta = plan + tg.o_meta;
   if (pass >= meta[kR3MetaPasses]) return;
-  const int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p64 >= meta[kR3MetaEntries]) return;
-  const int p = (int)p64;
+  const int total = meta[kR3MetaEntries];
   const int* src = plan + ((pass & 1) ? tg.o_scratch : tg.o_perm);
   int* dst = plan + ((pass & 1) ? tg.o_perm : tg.o_scratch);
   const int* start = plan + tg.o_start;
-  int lo = 0, hi = tg.NB;  // last tile t with start[t] <= p
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (start[mid] <= p) lo = mid;
-    else hi = mid;
-  }
-  const int s0 = start[lo], V = start[lo + 1] - s0, o = p - s0;
   const int R = kR3SortChunk << pass;
-  const int key = src[p];
-  if (V <= R) {
-    dst[p] = key;
-    return;
+  for (int64_t p64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p64 < total;
+       p64 += (int64_t)gridDim.x * blockDim.x) {
+    const int p = (int)p64;
+    int lo = 0, hi = tg.NB;  // last tile t with start[t] <= p
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (start[mid] <= p) lo = mid;
+      else hi = mid;
+    }
+    const int s0 = start[lo], V = start[lo + 1] - s0, o = p - s0;
+    const int key = src[p];
+    if (V <= R) {
+      dst[p] = key;
+      continue;
+    }
+    const int r = o / R, base = (r & ~1) * R;
+    int no;
+    if ((r & 1) == 0) no = o + count_less(src + s0, min(base + R, V), min(base + 2 * R, V), key);
+    else no = base + (o - base - R) + count_less(src + s0, base, base + R, key);
+    dst[s0 + no] = key;
   }
-  const int r = o / R, base = (r & ~1) * R;
-  int no;
-  if ((r & 1) == 0) no = o + count_less(src + s0, min(base + R, V), min(base + 2 * R, V), key);
-  else no = base + (o - base - R) + count_less(src + s0, base, base + R, key);
-  dst[s0 + no] = key;
 }
 
 // acc[C + j] += u * w[j] for the taps that land in the tile (C = the footprint's z offset)
@@ -2248,16 +2295,47 @@ __device__ __forceinline__ void r3_add_z(float2 (&acc)[kR3T2], const float2 u, c
   }
 }
 
+// Per-sample tap rows of the plan: for every sample and dimension the conjugated table
+// weights of its J taps (zero past J; dimension 0 also times the conjugated phase
+// exp(-i theta)) and the wrapped base cell of its footprint.
+__global__ void __launch_bounds__(256) reg3_sample_rows_kernel(const Geom<float> g, const TileGeom tg,
+                                                              const float* __restrict__ omega, int* __restrict__ plan,
+                                                              const CacheHdr* cache) {
+  if (cache_hit(cache)) return;
+  float2* sw = reinterpret_cast<float2*>(plan + tg.o_sw);
+  int* sb = plan + tg.o_sb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * g.M; i += (int64_t)gridDim.x * blockDim.x) {
+    const int d = (int)(i / g.M);
+    const int64_t m = i - d * g.M;
+    float2 w[6];
+    int mdx[6];
+    sample_taps_md<float, 6>(g, d, omega[d * g.M + m], w, mdx);
+    float2 ph = make_float2(1.f, 0.f);
+    if (d == 0) {
+      const float theta = sample_theta<float, 3>(g, omega, m);
+      ph = make_float2(cosf(theta), sinf(theta));
+    }
+    float2* row = sw + (d * g.M + m) * 6;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) row[j] = cmul_conj(make_float2(w[j].x, -w[j].y), ph);
+    sb[d * g.M + m] = mdx[0];
+  }
+}
+
 __global__ void __launch_bounds__(kR3Threads)
     spread_reg3_kernel(const Geom<float> g, const TileGeom tg, const float2* __restrict__ kdata,
                        const float* __restrict__ omega, const int* __restrict__ plan, float2* __restrict__ grid,
                        float2* __restrict__ pool) {
-  __shared__ __attribute__((aligned(16))) float2 W0[kR3Batch][6];
-  __shared__ __attribute__((aligned(16))) float2 W1[kR3Batch][6];
-  __shared__ __attribute__((aligned(16))) float2 W2[kR3Batch][6];
-  __shared__ float2 S[kR3Batch][kR3Rows];
-  __shared__ float2 Vy[kR3Batch][64];
-  __shared__ int R0[kR3Batch], R1[kR3Batch], R2[kR3Batch];
+  // LDS: the staged batch, and afterwards (same bytes) each wave's flush transpose
+  __shared__ __attribute__((aligned(16))) unsigned char smem[kR3LdsBytes];
+  float2(&W0)[kR3Batch][6] = *reinterpret_cast<float2(*)[kR3Batch][6]>(smem);
+  float2(&W1)[kR3Batch][6] = *reinterpret_cast<float2(*)[kR3Batch][6]>(smem + 3072);
+  float2(&W2)[kR3Batch][6] = *reinterpret_cast<float2(*)[kR3Batch][6]>(smem + 6144);
+  float2(&S)[kR3Batch][kR3Rows] = *reinterpret_cast<float2(*)[kR3Batch][kR3Rows]>(smem + 9216);
+  float2(&Vy)[kR3Batch][64] = *reinterpret_cast<float2(*)[kR3Batch][64]>(smem + 13312);
+  int* R0 = reinterpret_cast<int*>(smem + 46080);
+  int* R1 = R0 + kR3Batch;
+  int* R2 = R1 + kR3Batch;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int nchunk = tg.nchunk;
@@ -2285,44 +2363,58 @@ __global__ void __launch_bounds__(kR3Threads)
 #pragma unroll
   for (int z = 0; z < kR3T2; ++z) acc[z] = make_float2(0.f, 0.f);
 
+  // Staging reads are prefetched into registers: the sample indices two batches ahead,
+  // the plan's tap rows and the data one batch ahead, so that a batch's global loads are
+  // in flight while the previous batch is walked.  Thread (part, e) stages entry e:
+  // parts 0..2 the tap row of dimension `part`, every part the data of coil row `part`.
+  const int e_s = threadIdx.x & 63, part = threadIdx.x >> 6;
+  const float4* sw4 = reinterpret_cast<const float4*>(plan + tg.o_sw);
+  const int* sb = plan + tg.o_sb;
+  auto load_m = [&](int b) { return b + e_s < i1 ? perm[b + e_s] : -1; };
+  float4 wv[3];
+  int bmv = 0;
+  float2 kdv = make_float2(0.f, 0.f);
+  auto load_vals = [&](int m) {
+    if (m < 0) return;
+    if (part < 3) {
+      const float4* row = sw4 + ((int64_t)part * g.M + m) * 3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) wv[k] = row[k];
+      bmv = sb[part * g.M + m];
+    }
+    if (part < ncc) kdv = kdata[(rbase + part) * g.M + m];
+  };
+  int m_cur = load_m(i0);
+  int m_nxt = load_m(i0 + kR3Batch);
+  load_vals(m_cur);
+
   for (int base = i0; base < i1; base += kR3Batch) {
-    const int nb = min(kR3Batch, i1 - base);
     {
-      // stage: thread (part, e) computes the weight row of dimension `part` (parts 0..2)
-      // and the data of coil row `part` times the sample's phase, of entry e
-      const int e = threadIdx.x & 63, part = threadIdx.x >> 6;
-      const bool live = e < nb;
-      const int m = live ? perm[base + e] : 0;
+      const int e = e_s;
+      const bool live = m_cur >= 0;
       if (part < 3) {
-        float2(*Wd)[6] = part == 0 ? W0 : (part == 1 ? W1 : W2);
+        float4(*Wd)[3] = reinterpret_cast<float4(*)[3]>(part == 0 ? W0 : (part == 1 ? W1 : W2));
         int* Rd = part == 0 ? R0 : (part == 1 ? R1 : R2);
-        float2 w[6];
-        int mdx[6];
+        // footprint offset in the tile: taps run from the base cell, or from its wrap
+        // below the grid start when the tile lies past the grid's end
         int r = -100;  // no tap of a dead lane lands in the tile
         if (live) {
-          sample_taps_md<float, 6>(g, part, omega[part * g.M + m], w, mdx);
-#pragma unroll
-          for (int j = 0; j < 6; ++j) {
-            const int l = mdx[j] - org[part];
-            if (mdx[j] >= 0 && l >= 0 && l < tg.T[part]) r = l - j;  // one run per tile
-          }
-        } else {
-#pragma unroll
-          for (int j = 0; j < 6; ++j) w[j] = make_float2(0.f, 0.f);
+          r = bmv - org[part];
+          if (r >= tg.T[part]) r -= tg.K[part];
         }
 #pragma unroll
-        for (int j = 0; j < 6; ++j) Wd[e][j] = make_float2(w[j].x, -w[j].y);  // zero past J
+        for (int k = 0; k < 3; ++k) Wd[e][k] = live ? wv[k] : make_float4(0.f, 0.f, 0.f, 0.f);
         Rd[e] = r;
       }
-      float2 sv = make_float2(0.f, 0.f);
-      if (live && part < ncc) {
-        const float theta = sample_theta<float, 3>(g, omega, m);
-        const float2 ph = make_float2(cosf(theta), sinf(theta));
-        sv = cmul_conj(kdata[(rbase + part) * g.M + m], ph);
-      }
-      S[e][part] = sv;
+      S[e][part] = (live && part < ncc) ? kdv : make_float2(0.f, 0.f);
     }
-    __syncthreads();
+    // next batch's loads, in flight during this batch's walk
+    const int m_nn = base + 2 * kR3Batch < i1 ? load_m(base + 2 * kR3Batch) : -1;
+    kdv = make_float2(0.f, 0.f);
+    load_vals(m_nxt);
+    m_cur = m_nxt;
+    m_nxt = m_nn;
+    lds_barrier();  // LDS only: the prefetch loads stay in flight
     // data times the y weight for every (entry, y, row)
 #pragma unroll
     for (int k = 0; k < (kR3Batch * 64) / kR3Threads; ++k) {
@@ -2333,7 +2425,7 @@ __global__ void __launch_bounds__(kR3Threads)
       if (j1 >= 0 && j1 < 6) v = cmul(S[e][q], W1[e][j1]);
       Vy[e][l] = v;
     }
-    __syncthreads();
+    lds_barrier();  // LDS only: the prefetch loads stay in flight
     // walk: entries that reach this wave's plane, by z offset
     const int r0l = R0[lane], r2l = R2[lane];
     const int jxl = wave - r0l;
@@ -2355,50 +2447,45 @@ __global__ void __launch_bounds__(kR3Threads)
     TKBN_R3_GROUP(10) TKBN_R3_GROUP(11) TKBN_R3_GROUP(12) TKBN_R3_GROUP(13) TKBN_R3_GROUP(14)
     TKBN_R3_GROUP(15)
 #undef TKBN_R3_GROUP
-    __syncthreads();
+    lds_barrier();  // LDS only: the prefetch loads stay in flight
   }
 
-  const int y = lane >> 3, q = lane & 7;
-  if (split) {
-    // partial tile of this piece: slot (wi, chunk), element ((q * 8 + x) * 8 + y) * 16 + z
-    float2* dst = pool + ((int64_t)wi * nchunk + chunk) * kR3SlotFloats2 + ((q * kR3T0 + wave) * kR3T1 + y) * kR3T2;
+  // Flush through LDS (each wave its own region, after the loop's last barrier): lane
+  // c's z column goes out as 16 consecutive cells, four columns per store instruction.
+  float2* fl = reinterpret_cast<float2*>(smem) + wave * (64 * kR3FlushPitch);
 #pragma unroll
-    for (int z = 0; z < kR3T2; ++z) dst[z] = acc[z];
-    return;
-  }
-  const int k0 = org[0] + wave, k1 = org[1] + y;
-  if (q < ncc && k0 < g.K[0] && k1 < g.K[1]) {
-    float2* dst = grid + (rbase + q) * g.G + (int64_t)k0 * g.stride[0] + (int64_t)k1 * g.stride[1] + org[2];
+  for (int z = 0; z < kR3T2; ++z) fl[lane * kR3FlushPitch + z] = acc[z];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int z = lane & (kR3T2 - 1);
+  const int k0 = org[0] + wave, k2 = org[2] + z;
+  // partial tile of a piece: slot (wi, chunk), element ((x * 8 + y) * 8 + row) * 16 + z
+  float2* slot = pool + ((int64_t)wi * nchunk + chunk) * kR3SlotFloats2 + wave * 64 * kR3T2;
 #pragma unroll
-    for (int z = 0; z < kR3T2; ++z)
-      if (org[2] + z < g.K[2]) dst[z] = acc[z];
+  for (int i = 0; i < 64 / 4; ++i) {
+    const int c = 4 * i + (lane >> 4);  // source lane: y = c / 8, row = c % 8
+    const float2 v = fl[c * kR3FlushPitch + z];
+    if (split) {
+      slot[c * kR3T2 + z] = v;
+    } else {
+      const int yc = c >> 3, qc = c & 7, k1 = org[1] + yc;
+      if (qc < ncc && k0 < g.K[0] && k1 < g.K[1] && k2 < g.K[2])
+        grid[(rbase + qc) * g.G + (int64_t)k0 * g.stride[0] + (int64_t)k1 * g.stride[1] + k2] = v;
+    }
   }
 }
 
-// Hot tile h, coil chunk blockIdx.y: the sum of its pieces' partial tiles, in piece order.
-__global__ void __launch_bounds__(256) reg3_combine_kernel(const Geom<float> g, const TileGeom tg,
-                                                          const int* __restrict__ plan,
-                                                          const float2* __restrict__ pool, float2* __restrict__ grid) {
-  const int* meta = plan + tg.o_meta;
-  const int h = blockIdx.x;
-  if (h >= meta[kR3MetaHotTiles]) return;
-  const int nchunk = tg.nchunk, chunk = blockIdx.y;
-  const int P = meta[kR3MetaP];
-  const int t = plan[tg.o_hot + h];
-  const int w0 = plan[tg.o_wstart + t];
-  const int V = plan[tg.o_start + t + 1] - plan[tg.o_start + t];
-  const int np = (V + P - 1) / P;
-  const int64_t nrows = g.B * g.C;
-  const int64_t rbase = (int64_t)chunk * kR3Rows;
-  const int ncc = (int)(rbase + kR3Rows < nrows ? kR3Rows : nrows - rbase);
-  int org[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) org[d] = ((t / tg.bstride[d]) % tg.nb[d]) * tg.T[d];
-  for (int idx = threadIdx.x; idx < kR3SlotFloats2; idx += blockDim.x) {
-    const int z = idx & (kR3T2 - 1), y = (idx / kR3T2) & (kR3T1 - 1), x = (idx / (kR3T2 * kR3T1)) & (kR3T0 - 1);
-    const int q = idx / kR3Cells;
+// element idx of a hot tile's chunk: the sum of its pieces' values, in piece order
+__device__ __forceinline__ void combine_unit(const Geom<float>& g, const float2* __restrict__ pool,
+                                             float2* __restrict__ grid, int w0, int np, int nchunk, int chunk,
+                                             int64_t rbase, int ncc, const int (&org)[3], int idx) {
+  {
+    // slot element ((x * 8 + y) * 8 + row) * 16 + z
+    const int z = idx & (kR3T2 - 1), q = (idx / kR3T2) & (kR3Rows - 1), y = (idx / (kR3T2 * kR3Rows)) & (kR3T1 - 1);
+    const int x = idx / (kR3T2 * kR3Rows * kR3T1);
     const int k0 = org[0] + x, k1 = org[1] + y, k2 = org[2] + z;
-    if (q >= ncc || k0 >= g.K[0] || k1 >= g.K[1] || k2 >= g.K[2]) continue;
+    if (q >= ncc || k0 >= g.K[0] || k1 >= g.K[1] || k2 >= g.K[2]) return;
     float2 s = pool[((int64_t)w0 * nchunk + chunk) * kR3SlotFloats2 + idx];
     for (int k = 1; k < np; ++k) {
       const float2 v = pool[((int64_t)(w0 + k) * nchunk + chunk) * kR3SlotFloats2 + idx];
@@ -2406,6 +2493,33 @@ __global__ void __launch_bounds__(256) reg3_combine_kernel(const Geom<float> g, 
       s.y = s.y + v.y;
     }
     grid[(rbase + q) * g.G + (int64_t)k0 * g.stride[0] + (int64_t)k1 * g.stride[1] + k2] = s;
+  }
+}
+
+// Every hot tile's coil chunks, 256 elements per unit: the sum of the tile's pieces.
+__global__ void __launch_bounds__(256) reg3_combine_kernel(const Geom<float> g, const TileGeom tg,
+                                                          const int* __restrict__ plan,
+                                                          const float2* __restrict__ pool, float2* __restrict__ grid) {
+  const int* meta = plan + tg.o_meta;
+  const int nhot = meta[kR3MetaHotTiles];
+  const int nchunk = tg.nchunk;
+  const int P = meta[kR3MetaP];
+  const int64_t nrows = g.B * g.C;
+  constexpr int kSlices = (int)(kR3SlotFloats2 / 256);  // 256 elements per unit
+  for (int64_t unit = blockIdx.x; unit < (int64_t)nhot * nchunk * kSlices; unit += gridDim.x) {
+    const int slice = (int)(unit % kSlices);
+    const int chunk = (int)((unit / kSlices) % nchunk);
+    const int h = (int)(unit / ((int64_t)kSlices * nchunk));
+    const int t = plan[tg.o_hot + h];
+    const int w0 = plan[tg.o_wstart + t];
+    const int V = plan[tg.o_start + t + 1] - plan[tg.o_start + t];
+    const int np = (V + P - 1) / P;
+    const int64_t rbase = (int64_t)chunk * kR3Rows;
+    const int ncc = (int)(rbase + kR3Rows < nrows ? kR3Rows : nrows - rbase);
+    int org[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) org[d] = ((t / tg.bstride[d]) % tg.nb[d]) * tg.T[d];
+    combine_unit(g, pool, grid, w0, np, nchunk, chunk, rbase, ncc, org, slice * 256 + (int)threadIdx.x);
   }
 }
 
@@ -2437,8 +2551,10 @@ static int reg3_spread_launch(const tkbn_interp_desc* d, bool det, const void* k
   }
   const float* om = static_cast<const float*>(omega);
   if (cache) {
-    const dim3 gc((unsigned)std::max<int64_t>(1, ceil_div(g.M, 256)));
+    const dim3 gc(bin_grid(g.M));
     if (g.M > 0) {
+      reg3_sample_rows_kernel<<<dim3(bin_grid(3 * g.M)), dim3(256), 0, s>>>(g, tg, om, plan, cache);
+      TKBN_LAUNCH_CHECK("reg3_sample_rows_kernel");
       bin_count_kernel<float, 3><<<gc, dim3(256), 0, s>>>(g, tg, om, plan, cache);
       TKBN_LAUNCH_CHECK("bin_count_kernel<reg3>");
     }
@@ -2450,10 +2566,12 @@ static int reg3_spread_launch(const tkbn_interp_desc* d, bool det, const void* k
     }
     if (det && g.M > 0) {
       const int64_t nsort = tg.NB + tg.maxent / kR3SortChunk + 1;
-      reg3_sort_chunks_kernel<<<dim3((unsigned)nsort), dim3(1024), 0, s>>>(tg, plan, cache);
+      reg3_sort_chunks_kernel<<<dim3((unsigned)std::min<int64_t>(nsort, 1024)), dim3(1024), 0, s>>>(tg, plan, cache);
       TKBN_LAUNCH_CHECK("reg3_sort_chunks_kernel");
-      const dim3 gm((unsigned)ceil_div(tg.maxent, 256));
-      for (int pass = 0; ((int64_t)kR3SortChunk << pass) < tg.maxent; ++pass) {
+      const dim3 gm(bin_grid(tg.maxent));
+      // a tile holds at most M entries (one per sample)
+      const int64_t vbound = std::min<int64_t>(tg.maxent, g.M);
+      for (int pass = 0; ((int64_t)kR3SortChunk << pass) < vbound; ++pass) {
         reg3_merge_kernel<<<gm, dim3(256), 0, s>>>(tg, plan, cache, pass);
         TKBN_LAUNCH_CHECK("reg3_merge_kernel");
       }
@@ -2464,9 +2582,7 @@ static int reg3_spread_launch(const tkbn_interp_desc* d, bool det, const void* k
       g, tg, static_cast<const float2*>(kdata), om, plan, static_cast<float2*>(grid), pool);
   profile_after(s);
   TKBN_LAUNCH_CHECK("spread_reg3_kernel");
-  const int64_t hot_bound = std::max<int64_t>(1, std::min<int64_t>(tg.NB, tg.pool_slots / tg.nchunk));
-  reg3_combine_kernel<<<dim3((unsigned)hot_bound, (unsigned)tg.nchunk), dim3(256), 0, s>>>(
-      g, tg, plan, pool, static_cast<float2*>(grid));
+  reg3_combine_kernel<<<dim3(2048), dim3(256), 0, s>>>(g, tg, plan, pool, static_cast<float2*>(grid));
   TKBN_LAUNCH_CHECK("reg3_combine_kernel");
   note_kernel(det ? "spread_reg3_kernel<deterministic>" : "spread_reg3_kernel");
   return TKBN_OK;
@@ -2524,7 +2640,7 @@ int fwd_tiled_launch(const Geom<T>& g, const TileGeom& tg, const void* grid, con
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t zdim = g.batched ? g.B : 1;
   if (cache) {  // no header: the caller reuses the plan
-    dim3 gc((unsigned)ceil_div(g.M, 256), 1, (unsigned)zdim);
+    dim3 gc(bin_grid(g.M), 1, (unsigned)zdim);
     bin_count_kernel<T, D><<<gc, dim3(256), 0, s>>>(g, tg, static_cast<const T*>(omega), ws, cache);
     TKBN_LAUNCH_CHECK("bin_count_kernel<own>");
     plan_scan_kernel<<<dim3((unsigned)zdim), dim3(1024), 0, s>>>(tg, ws, cache);
