@@ -683,6 +683,75 @@ __device__ __forceinline__ int64_t xcd_work_item(int64_t b) {
 //   (wrapped) for the chunk's rows, then every sample based in the tile reads its
 //   J^d neighbourhood from LDS.
 // ---------------------------------------------------------------------------
+// acc += a * b with four fused multiply-adds into the accumulator
+template <typename C>
+__device__ __forceinline__ void cmac_fma(C& acc, const C a, const C b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+
+// Sum over the J^D taps of one sample from the staged region L (nested: the last
+// dimension's row first).  FULL: every numpoints is JM, so the loads of a row carry no
+// guards and are issued together.
+template <typename T, int D, int JM, bool FULL>
+__device__ __forceinline__ cplx<T> fwd_taps(const Geom<T>& g, const TileGeom& tg, const cplx<T>* L,
+                                            const cplx<T> (&w)[D][JM], const int (&loc0)[D]) {
+  using C2 = cplx<T>;
+  const C2 zero = make_c<T>(0, 0);
+  C2 acc = zero;
+  if constexpr (D == 1) {
+    C2 v[JM];
+#pragma unroll
+    for (int j = 0; j < JM; ++j) v[j] = (FULL || j < g.J[0]) ? L[loc0[0] + j] : zero;
+#pragma unroll
+    for (int j = 0; j < JM; ++j) cmac_fma(acc, w[0][j], v[j]);
+  } else if constexpr (D == 2) {
+#pragma unroll
+    for (int j0 = 0; j0 < JM; ++j0) {
+      if (FULL || j0 < g.J[0]) {
+        const C2* rp = L + (loc0[0] + j0) * tg.rstride[0] + loc0[1];
+        C2 v[JM];
+#pragma unroll
+        for (int j1 = 0; j1 < JM; ++j1) v[j1] = (FULL || j1 < g.J[1]) ? rp[j1] : zero;
+        C2 part = zero;
+#pragma unroll
+        for (int j1 = 0; j1 < JM; ++j1) cmac_fma(part, w[1][j1], v[j1]);
+        cmac_fma(acc, w[0][j0], part);
+      }
+    }
+  } else {
+    auto plane = [&](int j0) {  // sum over the (j1, j2) taps of plane j0
+      C2 p0 = zero;
+#pragma unroll
+      for (int j1 = 0; j1 < JM; ++j1) {
+        if (FULL || j1 < g.J[1]) {
+          const C2* rp = L + (loc0[0] + j0) * tg.rstride[0] + (loc0[1] + j1) * tg.rstride[1] + loc0[2];
+          C2 v[JM];
+#pragma unroll
+          for (int j2 = 0; j2 < JM; ++j2) v[j2] = (FULL || j2 < g.J[2]) ? rp[j2] : zero;
+          C2 p1 = zero;
+#pragma unroll
+          for (int j2 = 0; j2 < JM; ++j2) cmac_fma(p1, w[2][j2], v[j2]);
+          cmac_fma(p0, w[1][j1], p1);
+        }
+      }
+      return p0;
+    };
+    if constexpr (JM <= 6) {
+#pragma unroll
+      for (int j0 = 0; j0 < JM; ++j0)
+        if (FULL || j0 < g.J[0]) cmac_fma(acc, w[0][j0], plane(j0));
+    } else {
+#pragma unroll 1
+      for (int j0 = 0; j0 < JM; ++j0)
+        if (FULL || j0 < g.J[0]) cmac_fma(acc, w[0][j0], plane(j0));
+    }
+  }
+  return acc;
+}
+
 // One work item: tile piece wi, chunk yc of tg.CC coil rows, per-batch plan b (blockIdx
 // of interp_fwd_tiled_kernel, or a batch kernel's decoded item).
 template <typename T, int D, int JM>
@@ -733,6 +802,9 @@ __device__ __forceinline__ void fwd_tiled_item(const Geom<T>& g, const TileGeom&
   const int* perm = plan + tg.o_perm;
   const int cstart = plan[tg.o_start + t];
   const T* om = omega + (g.batched ? b * D * g.M : 0);
+  bool full = true;  // every numpoints is JM: unguarded taps
+#pragma unroll
+  for (int d = 0; d < D; ++d) full = full && g.J[d] == JM;
   for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
     const int m = perm[cstart + i];
     C2 w[D][JM];
@@ -749,43 +821,7 @@ __device__ __forceinline__ void fwd_tiled_item(const Geom<T>& g, const TileGeom&
     const C2 phase = make_c<T>(cos(theta), sin(theta));
     for (int c = 0; c < ncc; ++c) {
       const C2* L = lds + c * rc;
-      C2 acc = make_c<T>(0, 0);
-      if constexpr (D == 1) {
-#pragma unroll
-        for (int j0 = 0; j0 < JM; ++j0)
-          if (j0 < g.J[0]) cmac(acc, w[0][j0], L[loc0[0] + j0]);
-      } else if constexpr (D == 2) {
-#pragma unroll
-        for (int j0 = 0; j0 < JM; ++j0) {
-          if (j0 < g.J[0]) {
-            const C2* rp = L + (loc0[0] + j0) * tg.rstride[0] + loc0[1];
-            C2 part = make_c<T>(0, 0);
-#pragma unroll
-            for (int j1 = 0; j1 < JM; ++j1)
-              if (j1 < g.J[1]) cmac(part, w[1][j1], rp[j1]);
-            cmac(acc, w[0][j0], part);
-          }
-        }
-      } else {
-#pragma unroll 1
-        for (int j0 = 0; j0 < JM; ++j0) {
-          if (j0 < g.J[0]) {
-            C2 p0 = make_c<T>(0, 0);
-#pragma unroll
-            for (int j1 = 0; j1 < JM; ++j1) {
-              if (j1 < g.J[1]) {
-                const C2* rp = L + (loc0[0] + j0) * tg.rstride[0] + (loc0[1] + j1) * tg.rstride[1] + loc0[2];
-                C2 p1 = make_c<T>(0, 0);
-#pragma unroll
-                for (int j2 = 0; j2 < JM; ++j2)
-                  if (j2 < g.J[2]) cmac(p1, w[2][j2], rp[j2]);
-                cmac(p0, w[1][j1], p1);
-              }
-            }
-            cmac(acc, w[0][j0], p0);
-          }
-        }
-      }
+      const C2 acc = full ? fwd_taps<T, D, JM, true>(g, tg, L, w, loc0) : fwd_taps<T, D, JM, false>(g, tg, L, w, loc0);
       kdata[(rowbase + r0 + c) * g.M + m] = cmul(acc, phase);
     }
   }
