@@ -109,6 +109,85 @@ __global__ void __launch_bounds__(256)
 // For the adjoint's gradient the same formula applies with X = upstream grid
 // gradient and G = the adjoint's input samples.
 // ---------------------------------------------------------------------------
+template <typename C>
+__device__ __forceinline__ void cmac_fma(C& acc, const C a, const C b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+
+// S0 = sum_taps w X and S_d = sum_taps (dw_d prod_{d' != d} w_d') X over one row's taps
+// (nested sums, the last dimension innermost; FULL: no numpoints guards).
+template <typename T, int D, int JM, bool FULL>
+__device__ __forceinline__ void domega_sums(const Geom<T>& g, const cplx<T>* gp, const cplx<T> (&w)[D][JM],
+                                            const cplx<T> (&dw)[D][JM], const int64_t (&off)[D][JM], cplx<T>& S0,
+                                            cplx<T> (&S)[D]) {
+  using C2 = cplx<T>;
+  const C2 zero = make_c<T>(0, 0);
+  S0 = zero;
+#pragma unroll
+  for (int d = 0; d < D; ++d) S[d] = zero;
+  if constexpr (D == 1) {
+    C2 x[JM];
+#pragma unroll
+    for (int j0 = 0; j0 < JM; ++j0) x[j0] = (FULL || j0 < g.J[0]) ? gp[off[0][j0]] : zero;
+#pragma unroll
+    for (int j0 = 0; j0 < JM; ++j0) {
+      cmac_fma(S0, w[0][j0], x[j0]);
+      cmac_fma(S[0], dw[0][j0], x[j0]);
+    }
+  } else if constexpr (D == 2) {
+#pragma unroll
+    for (int j0 = 0; j0 < JM; ++j0) {
+      if (FULL || j0 < g.J[0]) {
+        const C2* rp = gp + off[0][j0];
+        C2 x[JM];
+#pragma unroll
+        for (int j1 = 0; j1 < JM; ++j1) x[j1] = (FULL || j1 < g.J[1]) ? rp[off[1][j1]] : zero;
+        C2 P = zero, P1 = zero;
+#pragma unroll
+        for (int j1 = 0; j1 < JM; ++j1) {
+          cmac_fma(P, w[1][j1], x[j1]);
+          cmac_fma(P1, dw[1][j1], x[j1]);
+        }
+        cmac_fma(S0, w[0][j0], P);
+        cmac_fma(S[0], dw[0][j0], P);
+        cmac_fma(S[1], w[0][j0], P1);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j0 = 0; j0 < JM; ++j0) {
+      if (FULL || j0 < g.J[0]) {
+        C2 Q = zero, Q1 = zero, Q2 = zero;
+#pragma unroll
+        for (int j1 = 0; j1 < JM; ++j1) {
+          if (FULL || j1 < g.J[1]) {
+            const C2* rp = gp + off[0][j0] + off[1][j1];
+            C2 x[JM];
+#pragma unroll
+            for (int j2 = 0; j2 < JM; ++j2) x[j2] = (FULL || j2 < g.J[2]) ? rp[off[2][j2]] : zero;
+            C2 P = zero, P2 = zero;
+#pragma unroll
+            for (int j2 = 0; j2 < JM; ++j2) {
+              cmac_fma(P, w[2][j2], x[j2]);
+              cmac_fma(P2, dw[2][j2], x[j2]);
+            }
+            cmac_fma(Q, w[1][j1], P);
+            cmac_fma(Q1, dw[1][j1], P);
+            cmac_fma(Q2, w[1][j1], P2);
+          }
+        }
+        cmac_fma(S0, w[0][j0], Q);
+        cmac_fma(S[0], dw[0][j0], Q);
+        cmac_fma(S[1], w[0][j0], Q1);
+        cmac_fma(S[2], w[0][j0], Q2);
+      }
+    }
+  }
+}
+
 template <typename T, int D, int JM>
 __global__ void __launch_bounds__(256)
     interp_domega_kernel(const Geom<T> g, const cplx<T>* __restrict__ grid, const cplx<T>* __restrict__ gout,
@@ -131,67 +210,14 @@ __global__ void __launch_bounds__(256)
   for (int d = 0; d < D; ++d) acc[d] = 0;
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
   const int64_t rowbase = g.batched ? b * g.C : 0;
+  bool full = true;  // every numpoints is JM: unguarded taps, row loads issued together
+#pragma unroll
+  for (int d = 0; d < D; ++d) full = full && g.J[d] == JM;
   for (int64_t r = 0; r < nrows; ++r) {
     const C2* gp = grid + (rowbase + r) * g.G;
-    C2 S0 = make_c<T>(0, 0);
-    C2 S[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) S[d] = make_c<T>(0, 0);
-    if constexpr (D == 1) {
-#pragma unroll
-      for (int j0 = 0; j0 < JM; ++j0)
-        if (j0 < g.J[0]) {
-          const C2 x = gp[off[0][j0]];
-          cmac(S0, w[0][j0], x);
-          cmac(S[0], dw[0][j0], x);
-        }
-    } else if constexpr (D == 2) {
-#pragma unroll
-      for (int j0 = 0; j0 < JM; ++j0) {
-        if (j0 < g.J[0]) {
-          const C2* rp = gp + off[0][j0];
-          C2 P = make_c<T>(0, 0), P1 = make_c<T>(0, 0);
-#pragma unroll
-          for (int j1 = 0; j1 < JM; ++j1)
-            if (j1 < g.J[1]) {
-              const C2 x = rp[off[1][j1]];
-              cmac(P, w[1][j1], x);
-              cmac(P1, dw[1][j1], x);
-            }
-          cmac(S0, w[0][j0], P);
-          cmac(S[0], dw[0][j0], P);
-          cmac(S[1], w[0][j0], P1);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int j0 = 0; j0 < JM; ++j0) {
-        if (j0 < g.J[0]) {
-          C2 Q = make_c<T>(0, 0), Q1 = make_c<T>(0, 0), Q2 = make_c<T>(0, 0);
-#pragma unroll
-          for (int j1 = 0; j1 < JM; ++j1) {
-            if (j1 < g.J[1]) {
-              const C2* rp = gp + off[0][j0] + off[1][j1];
-              C2 P = make_c<T>(0, 0), P2 = make_c<T>(0, 0);
-#pragma unroll
-              for (int j2 = 0; j2 < JM; ++j2)
-                if (j2 < g.J[2]) {
-                  const C2 x = rp[off[2][j2]];
-                  cmac(P, w[2][j2], x);
-                  cmac(P2, dw[2][j2], x);
-                }
-              cmac(Q, w[1][j1], P);
-              cmac(Q1, dw[1][j1], P);
-              cmac(Q2, w[1][j1], P2);
-            }
-          }
-          cmac(S0, w[0][j0], Q);
-          cmac(S[0], dw[0][j0], Q);
-          cmac(S[1], w[0][j0], Q1);
-          cmac(S[2], w[0][j0], Q2);
-        }
-      }
-    }
+    C2 S0, S[D];
+    if (full) domega_sums<T, D, JM, true>(g, gp, w, dw, off, S0, S);
+    else domega_sums<T, D, JM, false>(g, gp, w, dw, off, S0, S);
     // q = conj(G) * phase;  grad_d += Re(q * (i s_d S0 + S_d / gam_d))
     const C2 q = cmul_conj(phase, gout[(rowbase + r) * g.M + m]);
     const C2 qs = cmul(q, S0);
