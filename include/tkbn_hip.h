@@ -79,9 +79,12 @@ enum tkbn_dtype {
 
 enum tkbn_adj_mode {
   TKBN_ADJ_ATOMIC = 0,       /* sample-parallel spread, float atomics (order varies run to run) */
-  TKBN_ADJ_DETERMINISTIC = 1 /* binned spread with 64-bit fixed-point tile accumulators and no
-                                split tiles: the result does not depend on arrival order, so it
-                                is bitwise repeatable */
+  TKBN_ADJ_DETERMINISTIC = 1 /* bitwise repeatable spread: the sample-lane kernels accumulate
+                                64-bit fixed point in LDS tiles (order independent, no split
+                                tiles); the 3-D register-tile spread (complex64, >= 4 coil rows)
+                                sums in float in a fixed order (each tile's entries sorted by
+                                sample index when the plan is built, the pieces of a hot tile
+                                added in piece order) */
 };
 
 /*
@@ -197,8 +200,9 @@ TKBN_API int tkbn_interp_adj_noalloc(const tkbn_interp_desc* desc, const void* k
  * with one launch per stage over all problems.
  *
  * Every problem is complex64, of the same ndim and numpoints class (all <= 6 or all
- * <= 16), with an unbatched trajectory and fewer than 32 coil rows (the sample-lane
- * kernels; larger problems take the multi-coil kernels, one call each).  The caller
+ * <= 16), with an unbatched trajectory and fewer than 32 coil rows, or fewer than 4 in
+ * 3-D (the sample-lane kernels; larger problems take the multi-coil kernels, one call
+ * each).  The caller
  * first makes one tkbn_interp_fwd and one tkbn_interp_adj call per problem with its
  * two workspaces (this builds the plans; the forward needs its workspace), then:
  *   tkbn_batch_create     describes the problems once: their descriptors (`adj_descs`, or
