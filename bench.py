@@ -301,6 +301,7 @@ def bench_batched(dev, iters: int) -> dict:
         "speedup_graph_vs_loop": t_loop / t_graph,
         "autotune": {f"{k[0]}x{k[1]}": {d: {"tile": list(v[0][: len(k)]), "work_target": v[1]} for d, v in c.items()}
                      for k, c in bt.tuning.items()},
+        "batch_tuning": bt.batch_tuning,
         "max_rel_diff_vs_loop": diff,
         "fp64_oracle_rel_err": fp64,
     }

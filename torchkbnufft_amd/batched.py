@@ -156,8 +156,10 @@ class BatchedKbNufft(torch.nn.Module):
         self._det = deterministic
         self._built = False
         self._batch: Optional[_BatchHandle] = None
-        # per image size: {"fwd": (tile, target), "adj": (tile, target)}
+        # per image size: {"fwd": (tile, target), "adj": (tile, target)} (per-problem tuner)
         self.tuning: Dict[Tuple[int, ...], Dict[str, tuple]] = {}
+        # the setting the fused batch runs with, and the batch times that chose it
+        self.batch_tuning: Dict[str, object] = {}
 
     # ------------------------------------------------------------------ set-up
     def _interp_args(self, g: int):
@@ -181,16 +183,9 @@ class BatchedKbNufft(torch.nn.Module):
                 self._fwd_plans[q].set_tuning(*fwd)
                 self._adj_plans[q].set_tuning(*adj)
 
-    def _build(self) -> None:
-        """Build every problem's plans (one ordinary call per problem and direction into
-        scratch outputs), then the batch table when the problems qualify."""
-        det = _interp.deterministic_enabled() if self._det is None else bool(self._det)
-        self._det = det
-        if self._autotune:
-            self._tune(det)
-        fused = self._want_fused is not False and self.coils < 32
-        if self._want_fused and self.coils >= 32:
-            raise NotImplementedError("fused batches take fewer than 32 coil rows per problem")
+    def _build_plans(self, det: bool, fused: bool) -> None:
+        """Every problem's plans: one ordinary call per problem and direction, into scratch
+        outputs, through the problem's plan holders (and their tuning)."""
         for g, members in enumerate(self.groups):
             tabs, n_shift, numpoints, L, offsets = self._interp_args(g)
             gs = self._gs[g]
@@ -204,9 +199,88 @@ class BatchedKbNufft(torch.nn.Module):
                     self.kdata[p], self.ktrajs[p], tabs, n_shift, numpoints, L, offsets, gs, deterministic=det,
                     out=scratch_grid, plan=self._adj_plans[p],
                 )
+
+    def _build(self) -> None:
+        """Build every problem's plans, then the batch table when the problems qualify."""
+        det = _interp.deterministic_enabled() if self._det is None else bool(self._det)
+        self._det = det
+        if self._autotune:
+            self._tune(det)
+        fused = self._want_fused is not False and self.coils < 32
+        if self._want_fused and self.coils >= 32:
+            raise NotImplementedError("fused batches take fewer than 32 coil rows per problem")
+        self._build_plans(det, fused)
         if fused:
             self._batch = self._make_batch(det, required=bool(self._want_fused))
+            if self._batch is not None and self._autotune:
+                self._tune_batch(det)
         self._built = True
+
+    def _run_batch(self, direction: str) -> None:
+        b = self._batch
+        fn = _lib.load().tkbn_batch_interp_fwd if direction == "fwd" else _lib.load().tkbn_batch_interp_adj
+        with torch.cuda.device(self._device):
+            st = fn(b.ptr, b.table.data_ptr(), b.workspace.data_ptr(), b.workspace.numel(), _lib.stream_of(b.table))
+        _lib.check(st, f"tkbn_batch_interp_{direction}")
+
+    def _time_batch(self, reps: int = 5) -> float:
+        """Median ms of one batched gather + spread (HIP events, current stream)."""
+        ts = []
+        for i in range(reps + 2):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            self._run_batch("fwd")
+            self._run_batch("adj")
+            b.record()
+            b.synchronize()
+            if i >= 2:
+                ts.append(a.elapsed_time(b))
+        ts.sort()
+        return ts[len(ts) // 2]
+
+    def _tune_batch(self, det: bool) -> None:
+        """Per-problem choices can lose inside one launch (small work items multiply its
+        blocks), so the batch itself is timed under the per-problem choices and under a
+        few settings shared by all problems (library tiles, work targets 0 = the
+        library's), and the fastest is kept (another must be >= 3 % faster)."""
+        n = len(self.sizes)
+        own = [(self._fwd_plans[p].tile, self._fwd_plans[p].work_target, self._adj_plans[p].tile,
+                self._adj_plans[p].work_target) for p in range(n)]
+        required = bool(self._want_fused)
+        configs = [("per-problem", None)] + [
+            (f"shared: gather target {wf or 'default'}, spread target {wa or 'default'}", (wf, wa))
+            for wf in (0, 2048) for wa in (0, 1024, 2048)
+        ]
+        results = []
+        best = None
+        saved = [k.clone() for k in self.kdata]  # the timed gathers overwrite kdata (an adjoint's input)
+        for name, cfg in configs:
+            if cfg is not None:
+                for p in range(n):
+                    self._fwd_plans[p].set_tuning((0, 0, 0), cfg[0])
+                    self._adj_plans[p].set_tuning((0, 0, 0), cfg[1])
+                self._build_plans(det, True)
+                self._batch = self._make_batch(det, required)
+                if self._batch is None:
+                    continue
+            ms = self._time_batch()
+            results.append((name, round(ms, 4)))
+            if best is None or ms < 0.97 * best[1]:
+                best = (name, ms, cfg)
+        name, ms, cfg = best
+        for p in range(n):
+            if cfg is None:
+                self._fwd_plans[p].set_tuning(own[p][0], own[p][1])
+                self._adj_plans[p].set_tuning(own[p][2], own[p][3])
+            else:
+                self._fwd_plans[p].set_tuning((0, 0, 0), cfg[0])
+                self._adj_plans[p].set_tuning((0, 0, 0), cfg[1])
+        self._build_plans(det, True)
+        self._batch = self._make_batch(det, required)
+        for k, v in zip(self.kdata, saved):
+            k.copy_(v)
+        self.batch_tuning = {"choice": name, "ms": round(ms, 4), "tried": results}
 
     def _make_batch(self, det: bool, required: bool) -> Optional[_BatchHandle]:
         lib = _lib.load()
@@ -303,11 +377,7 @@ class BatchedKbNufft(torch.nn.Module):
             grids.append(_fft.fft_grid_fwd(self._img[g], None, op.scaling_coef, self._im[g], self._gs[g], 1.0,
                                            out=self._grid[g]))
         if self._batch is not None:
-            b = self._batch
-            with torch.cuda.device(self._device):
-                st = _lib.load().tkbn_batch_interp_fwd(b.ptr, b.table.data_ptr(), b.workspace.data_ptr(),
-                                                       b.workspace.numel(), _lib.stream_of(b.table))
-            _lib.check(st, "tkbn_batch_interp_fwd")
+            self._run_batch("fwd")
             return self.kdata
         for g, members in enumerate(self.groups):
             grid = grids[g]
@@ -327,11 +397,7 @@ class BatchedKbNufft(torch.nn.Module):
         if not self._built:
             self._build()
         if self._batch is not None:
-            b = self._batch
-            with torch.cuda.device(self._device):
-                st = _lib.load().tkbn_batch_interp_adj(b.ptr, b.table.data_ptr(), b.workspace.data_ptr(),
-                                                       b.workspace.numel(), _lib.stream_of(b.table))
-            _lib.check(st, "tkbn_batch_interp_adj")
+            self._run_batch("adj")
         for g, members in enumerate(self.groups):
             op = self.ops[g]
             im, gs = self._im[g], self._gs[g]
