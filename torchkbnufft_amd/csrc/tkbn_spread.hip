@@ -22,7 +22,10 @@
 // Deterministic mode accumulates 64-bit fixed point (integer adds are associative, so
 // the result does not depend on the order samples arrive in) and never splits tiles:
 // the output is bitwise repeatable.  The fixed-point scale is 2^e with
-// e = 60 - ceil(log2(2 * max|kdata| * candidates)), so no tile sum can overflow.
+// e = 60 - ceil(log2(2 * max|kdata| * candidates * wraps)), wraps = prod_d ceil(J_d / K_d),
+// so no tile sum can overflow for tables of modulus <= 1 (the Kaiser-Bessel tables are
+// normalised to 1).  Complex128 data keeps ~60 bits per tile sum: a quantum of
+// 2 max|kdata| V 2^-60 (about 1e-13 relative for a 5e4-entry tile).
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -430,9 +433,15 @@ __device__ __forceinline__ void spread_tiles_item(const Geom<T>& g, const TileGe
 
   double scale = 1.0;
   if constexpr (FIXED) {
-    const double bound = 2.0 * (double)amax * (double)(V > 0 ? V : 1);
+    // |re|, |im| of one tap <= 2 max(|re|, |im|) of the data times |w_0 .. w_{D-1}| <= 1
+    // (tables normalised to 1); a cell meets at most V entries, each up to
+    // prod_d ceil(J_d / K_d) times (footprints wider than a tiny grid wrap onto it)
+    double mult = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) mult *= (double)((g.J[d] + g.K[d] - 1) / g.K[d]);
+    const double bound = 2.0 * (double)amax * (double)(V > 0 ? V : 1) * mult;
     int e = 50;
-    if (bound > 0 && bound < 1e300) e = 60 - (ilogb(bound) + 1);
+    if (bound > 0) e = 60 - (ilogb(bound) + 1);  // amax is finite here (the float kernel takes the rest)
     e = e > 100 ? 100 : e;
     scale = ldexp(1.0, e);
   }
