@@ -610,9 +610,11 @@ def main() -> int:
     ap.add_argument("--stage-iters", type=int, default=20)
     ap.add_argument("--form-iters", type=int, default=10, help="iterations of the batched / gradient forms")
     ap.add_argument("--no-forms", action="store_true", help="skip the batched and gradient forms")
-    ap.add_argument("--workload", default="sense2d", choices=["sense2d", "kooshball3d", "copy512", "spmat", "batched"],
+    ap.add_argument("--workload", default="sense2d",
+                    choices=["sense2d", "kooshball3d", "copy512", "spmat", "batched", "grad"],
                     help="sense2d: the headline SENSE pair (default); kooshball3d: the 3-D deterministic case; "
                     "batched: the 64 uneven spiral problems only (fused batch, separate launches, plain loop); "
+                    "grad: the image + trajectory gradient form only; "
                     "copy512: a plain 512 MiB read + 512 MiB write kernel (calibrates rocprofv3 FETCH/WRITE_SIZE); "
                     "spmat: the SENSE batch with sparse-matrix interpolation only")
     args = ap.parse_args()
@@ -633,6 +635,9 @@ def main() -> int:
         return 0
     if args.workload == "batched":
         print(json.dumps(bench_batched(dev, args.form_iters)), flush=True)
+        return 0
+    if args.workload == "grad":
+        print(json.dumps(bench_grad(dev, args.form_iters)), flush=True)
         return 0
     if args.workload == "copy512":
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)  # 512 MiB
