@@ -385,6 +385,24 @@ struct AccOf<T, true> {
   using type = unsigned long long;
 };
 
+// x * scale as a two's-complement 64-bit integer.  float: the product is exact (scale is
+// a power of two, |x * scale| < 2^61), and its magnitude a splits exactly into
+// 2^32 * hi + lo with hi = floor(a / 2^32) and lo in [0, 2^32) (a's 24 significant bits
+// either all lie below 2^32, hi = 0, or lo holds those of them below 2^32), so two 32-bit
+// conversions replace a double round trip; rounded toward zero.  double: to nearest.
+__device__ __forceinline__ unsigned long long to_fixed(float x, float scale) {
+  const float v = x * scale;
+  const float a = fabsf(v);
+  const float hi = floorf(a * 0x1p-32f);
+  const float lo = fmaf(hi, -0x1p32f, a);
+  const unsigned long long u =
+      ((unsigned long long)(unsigned int)hi << 32) + (unsigned long long)(unsigned int)lo;
+  return v < 0.f ? 0ull - u : u;
+}
+__device__ __forceinline__ unsigned long long to_fixed(double x, double scale) {
+  return (unsigned long long)llrint(x * scale);
+}
+
 // FIXED: 64-bit fixed-point LDS accumulation (LDS integer atomics are much faster
 // than float ones; the sum is also order independent).  It exits when the max|kdata|
 // slot says a non-finite value is present; the float kernel launched with
@@ -445,6 +463,7 @@ __device__ __forceinline__ void spread_tiles_item(const Geom<T>& g, const TileGe
     e = e > 100 ? 100 : e;
     scale = ldexp(1.0, e);
   }
+  const T fscale = (T)scale;  // a power of two: x * fscale is exact (|x * fscale| < 2^61)
 
   const int* perm = plan + tg.o_perm;
   const T* om = omega + (g.batched ? b * D * g.M : 0);
@@ -470,8 +489,8 @@ __device__ __forceinline__ void spread_tiles_item(const Geom<T>& g, const TileGe
       A* aim = are + tg.tcells;
       auto add = [&](int cell, C2 x) {
         if constexpr (FIXED) {
-          atomicAdd(are + cell, (unsigned long long)llrint((double)x.x * scale));
-          atomicAdd(aim + cell, (unsigned long long)llrint((double)x.y * scale));
+          atomicAdd(are + cell, to_fixed(x.x, fscale));
+          atomicAdd(aim + cell, to_fixed(x.y, fscale));
         } else {
           atomicAdd(are + cell, x.x);
           atomicAdd(aim + cell, x.y);
