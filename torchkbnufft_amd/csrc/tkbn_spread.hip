@@ -68,6 +68,17 @@ struct TileGeom {
 // dynamic LDS per group; with the ~350 bytes of static LDS the total stays under 64 KiB
 constexpr int kLdsBudget = 60 * 1024;
 
+// Threads per group of the sample-lane kernels (single-problem and batched).  The gather
+// stages one region per group (at most two groups per CU), so 1024 lanes hide its LDS and
+// tap latency best (8-coil 320^3 kooshball forward 4.0 -> 2.8 ms, 64 uneven 2-D problems
+// 4.4 -> 4.0 ms per batched pair, 1-coil 256^2 pair 0.17 -> 0.15 ms).  The spread keeps
+// 256: more lanes per tile contend for the same LDS accumulators (the batched pair
+// 4.4 -> 5.4-6.0 ms with 512-1024).
+// (complex128 and numpoints above 6 keep 256: at 1024 threads their registers spill.)
+constexpr int spread_tiles_threads(int) { return 256; }
+template <typename T, int JM>
+constexpr int fwd_tiled_threads() { return sizeof(T) == 4 && JM <= 6 ? 1024 : 256; }
+
 template <typename T>
 __device__ __forceinline__ int64_t sample_base_wrapped(const Geom<T>& g, int d, T omega) {
 #pragma clang fp contract(off)
@@ -566,7 +577,7 @@ __device__ __forceinline__ void spread_tiles_item(const Geom<T>& g, const TileGe
 }
 
 template <typename T, bool FIXED, int D, int JM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(spread_tiles_threads(D))
     spread_tiles_kernel(const Geom<T> g, const TileGeom tg, const cplx<T>* __restrict__ kdata,
                         const T* __restrict__ omega, const int* __restrict__ ws, const void* __restrict__ maxabs_slot,
                         cplx<T>* __restrict__ grid, const int fallback) {
@@ -855,13 +866,8 @@ __device__ __forceinline__ void fwd_tiled_item(const Geom<T>& g, const TileGeom&
   }
 }
 
-// 3-D: 1024 threads per group (its staged region holds one coil row, so a CU holds two
-// groups; four times the lanes hide the region's LDS and the taps' latency: 4.0 -> 2.8 ms
-// for the 8-coil 320^3 kooshball forward, 512 threads 2.9 ms)
-constexpr int fwd_tiled_threads(int D) { return D == 3 ? 1024 : 256; }
-
 template <typename T, int D, int JM>
-__global__ void __launch_bounds__(fwd_tiled_threads(D))
+__global__ void __launch_bounds__((fwd_tiled_threads<T, JM>()))
     interp_fwd_tiled_kernel(const Geom<T> g, const TileGeom tg, const cplx<T>* __restrict__ grid,
                             const T* __restrict__ omega, const int* __restrict__ ws, cplx<T>* __restrict__ kdata) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -1806,13 +1812,13 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
     maxabs_kernel<T><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(static_cast<const T*>(kdata), n, om, nom,
                                                                    maxabs_slot);
     TKBN_LAUNCH_CHECK("maxabs_kernel");
-    spread_tiles_kernel<T, true, D, JM><<<grid3, dim3(256), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 0);
+    spread_tiles_kernel<T, true, D, JM><<<grid3, dim3(spread_tiles_threads(D)), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 0);
     TKBN_LAUNCH_CHECK("spread_tiles_kernel<fixed>");
-    spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(256), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 1);
+    spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(spread_tiles_threads(D)), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 1);
     TKBN_LAUNCH_CHECK("spread_tiles_kernel<non-finite>");
     note_kernel(det ? "spread_tiles_kernel<fixed,deterministic>" : "spread_tiles_kernel<fixed>");
   } else {
-    spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(256), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 0);
+    spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(spread_tiles_threads(D)), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 0);
     TKBN_LAUNCH_CHECK("spread_tiles_kernel");
     note_kernel("spread_tiles_kernel<float>");
   }
@@ -2719,7 +2725,7 @@ int fwd_tiled_launch(const Geom<T>& g, const TileGeom& tg, const void* grid, con
   }
   const size_t lds = (size_t)tg.CC * tg.rcells * sizeof(cplx<T>);
   interp_fwd_tiled_kernel<T, D, JM>
-      <<<dim3((unsigned)tg.maxwork, (unsigned)ceil_div(nrows, tg.CC), (unsigned)zdim), dim3(fwd_tiled_threads(D)),
+      <<<dim3((unsigned)tg.maxwork, (unsigned)ceil_div(nrows, tg.CC), (unsigned)zdim), dim3(fwd_tiled_threads<T, JM>()),
          lds, s>>>(g, tg, static_cast<const cplx<T>*>(grid), static_cast<const T*>(omega), ws,
                    static_cast<cplx<T>*>(kdata));
   TKBN_LAUNCH_CHECK("interp_fwd_tiled_kernel");
@@ -2894,7 +2900,7 @@ __device__ __forceinline__ int batch_find(const int* off, int nprob, int item) {
 }
 
 template <int D, int JM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__((fwd_tiled_threads<float, JM>()))
     batch_fwd_kernel(const BatchProb* __restrict__ probs, int nprob, const int* __restrict__ off) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int item = blockIdx.x;
@@ -2951,7 +2957,7 @@ __global__ void __launch_bounds__(256) batch_maxabs_kernel(const BatchProb* __re
 }
 
 template <bool FIXED, int D, int JM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(spread_tiles_threads(D))
     batch_spread_kernel(const BatchProb* __restrict__ probs, int nprob, const int* __restrict__ off, int fallback) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int item = blockIdx.x;
@@ -3126,7 +3132,8 @@ int tkbn_batch_interp_fwd(const tkbn_batch* b, const void* table, void* workspac
   TKBN_LAUNCH_CHECK("batch_offsets_kernel");
   if (b->bound_f == 0) return TKBN_OK;
   const dim3 grd((unsigned)b->bound_f);
-#define TKBN_BATCH_FWD(DD, JJ) batch_fwd_kernel<DD, JJ><<<grd, dim3(256), b->lds_f, s>>>(probs, b->nprob, off_f)
+#define TKBN_BATCH_FWD(DD, JJ) \
+  batch_fwd_kernel<DD, JJ><<<grd, dim3(fwd_tiled_threads<float, JJ>()), b->lds_f, s>>>(probs, b->nprob, off_f)
   if (b->jm == 6) {
     if (b->ndim == 1) TKBN_BATCH_FWD(1, 6);
     else if (b->ndim == 2) TKBN_BATCH_FWD(2, 6);
@@ -3166,7 +3173,7 @@ int tkbn_batch_interp_adj(const tkbn_batch* b, const void* table, void* workspac
   if (b->bound_a == 0) return TKBN_OK;
   const dim3 grd((unsigned)b->bound_a);
 #define TKBN_BATCH_ADJ(FX, DD, JJ, FB) \
-  batch_spread_kernel<FX, DD, JJ><<<grd, dim3(256), b->lds_a, s>>>(probs, b->nprob, off_a, FB)
+  batch_spread_kernel<FX, DD, JJ><<<grd, dim3(spread_tiles_threads(DD)), b->lds_a, s>>>(probs, b->nprob, off_a, FB)
 #define TKBN_BATCH_ADJ_PAIR(DD, JJ)      \
   do {                                   \
     TKBN_BATCH_ADJ(true, DD, JJ, 0);     \
