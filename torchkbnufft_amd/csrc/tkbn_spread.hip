@@ -1397,32 +1397,42 @@ struct SepLds {
 };
 static_assert(sizeof(SepLds) == 64, "staged record is 64 bytes");
 
+// Two staged entries a, b of one footprint column group at once (b may be the zero
+// dummy entry): ja / jb = the lane's tap row of each (any value; rows outside 0..J0-1
+// add nothing).  Branch-free, so the LDS reads of both entries are in flight together;
+// the adds into each accumulator keep the order a then b.
 template <int C0>
-__device__ __forceinline__ void sep_add_row_real(float2 (&acc)[kRegT1], const float2 u, const float (&w1)[6]) {
+__device__ __forceinline__ void sep_entry2(float2 (&acc)[kRegT1], const SepLds* ra, const SepLds* rb,
+                                           const float2 sa, const float2 sb, int ja, int jb, unsigned J0) {
+  float w1a[6], w1b[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    w1a[j] = ra->w1r[j];
+    w1b[j] = rb->w1r[j];
+  }
+  const float wa = ra->w0r[min(max(ja, 0), 5)], wb = rb->w0r[min(max(jb, 0), 5)];
+  const bool va = (unsigned)ja < J0, vb = (unsigned)jb < J0;
+  const float2 ua = va ? make_float2(sa.x * wa, sa.y * wa) : make_float2(0.f, 0.f);
+  const float2 ub = vb ? make_float2(sb.x * wb, sb.y * wb) : make_float2(0.f, 0.f);
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
     if (C0 + j >= 0 && C0 + j < kRegT1) {
-      acc[C0 + j].x = fmaf(u.x, w1[j], acc[C0 + j].x);
-      acc[C0 + j].y = fmaf(u.y, w1[j], acc[C0 + j].y);
+      acc[C0 + j].x = fmaf(ua.x, w1a[j], acc[C0 + j].x);
+      acc[C0 + j].y = fmaf(ua.y, w1a[j], acc[C0 + j].y);
+      acc[C0 + j].x = fmaf(ub.x, w1b[j], acc[C0 + j].x);
+      acc[C0 + j].y = fmaf(ub.y, w1b[j], acc[C0 + j].y);
     }
   }
-}
-
-template <int C0>
-__device__ __forceinline__ void sep_entry(float2 (&acc)[kRegT1], const SepLds* rec, const float2 s, int j0, int J0) {
-  float w1[6];
-#pragma unroll
-  for (int j = 0; j < 6; ++j) w1[j] = rec->w1r[j];
-  const float w0 = rec->w0r[min(max(j0, 0), 5)];
-  if (j0 >= 0 && j0 < J0) sep_add_row_real<C0>(acc, make_float2(s.x * w0, s.y * w0), w1);
 }
 
 __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_eu(6)))
     spread_sep_kernel(const Geom<float> g, const TileGeom tg, const SepGeom sg, const float2* __restrict__ kdata,
                       const float* __restrict__ omega, const int* __restrict__ ws, float2* __restrict__ grid,
                       int nchunk) {
-  __shared__ __attribute__((aligned(16))) float2 vals[kRegBatch * kRegPitch];
-  __shared__ __attribute__((aligned(16))) SepLds brec[kRegBatch];
+  // two staging buffers (batch k in buffer k & 1: one barrier per batch), each with a
+  // zero dummy entry kRegBatch; buffer 0 is also the flush's transposition space
+  __shared__ __attribute__((aligned(16))) float2 vals[2][(kRegBatch + 1) * kRegPitch];
+  __shared__ __attribute__((aligned(16))) SepLds brec[2][kRegBatch + 1];
   __shared__ float2 rph[kRegT0], cphs[kRegT1];  // conj(P_0) of the tile rows, conj(P_1) of its columns
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -1458,6 +1468,9 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
         reinterpret_cast<const float2*>(plan + sg.o_ptab[1])[min(o1 + (int)threadIdx.x - kRegT0, g.K[1] - 1)];
     cphs[threadIdx.x - kRegT0] = make_float2(p.x, -p.y);
   }
+  for (int i = threadIdx.x; i < 2 * kRegPitch; i += kRegThreads)
+    vals[i / kRegPitch][kRegBatch * kRegPitch + i % kRegPitch] = make_float2(0.f, 0.f);
+  if (threadIdx.x < 2 * 16) reinterpret_cast<int*>(&brec[threadIdx.x / 16][kRegBatch])[threadIdx.x % 16] = 0;
   lds_barrier();  // the flush reads them even when the tile has no entries
 
   float2 acc[kRegT1];
@@ -1494,10 +1507,13 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
     const int b1 = i0 + kRegBatch;
     if (b1 < i1 && lane < min(kRegBatch, i1 - b1)) mnext = perm[b1 + lane];
   }
-  for (int base = i0; base < i1; base += kRegBatch) {
+  int buf = 0;
+  for (int base = i0; base < i1; base += kRegBatch, buf ^= 1) {
     const int nb = min(kRegBatch, i1 - base);
+    float2* const vb = vals[buf];
+    SepLds* const bb = brec[buf];
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) vals[lane * kRegPitch + wave * kPer + q] = cmul(pv[q], pF);
+    for (int q = 0; q < kPer; ++q) vb[lane * kRegPitch + wave * kPer + q] = cmul(pv[q], pF);
     anyexc = anyexc || __ballot(pme < 0) != 0;
     // records: float4 q of entry e holds (me, rc, F) for q = 0, then w0[0..3],
     // w0[4..5] w1[0..1], w1[2..5]: the latter three are the first 48 bytes of SepLds
@@ -1505,14 +1521,16 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
       const int e = threadIdx.x >> 2, q = threadIdx.x & 3;
       if (q == 0) {
         const int rc = __float_as_int(prec.y);
-        brec[e].r0 = (rc & 255) - 8;
-        brec[e].c0 = ((rc >> 8) & 255) - 8;
+        bb[e].r0 = (rc & 255) - 8;
+        bb[e].c0 = ((rc >> 8) & 255) - 8;
       } else {
-        reinterpret_cast<float4*>(brec + e)[q - 1] = prec;
+        reinterpret_cast<float4*>(bb + e)[q - 1] = prec;
       }
     }
-    __syncthreads();
-    const int r0l = lane < nb ? brec[lane].r0 : 1000, c0l = lane < nb ? brec[lane].c0 : 1000;
+    // The other buffer was last read in the previous batch's walk, which every wave
+    // finished before this barrier; global loads stay in flight across it.
+    lds_barrier();
+    const int r0l = lane < nb ? bb[lane].r0 : 1000, c0l = lane < nb ? bb[lane].c0 : 1000;
     const int b1 = base + kRegBatch;
     if (b1 < i1) {
       prefetch(b1, mnext);
@@ -1525,10 +1543,17 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
   {                                                                                               \
     unsigned long long mk = __ballot(c0l == (C0)) & hits;                                         \
     while (mk) {                                                                                  \
-      const int c = __ffsll(mk) - 1;                                                              \
+      const int ca = __ffsll(mk) - 1;                                                             \
       mk &= mk - 1;                                                                               \
-      const int ja = wrow - __builtin_amdgcn_readlane(r0l, c);                                    \
-      sep_entry<C0>(acc, brec + c, vals[c * kRegPitch + coil], ja + hh, J0);                      \
+      int cb = kRegBatch;                                                                         \
+      if (mk) {                                                                                   \
+        cb = __ffsll(mk) - 1;                                                                     \
+        mk &= mk - 1;                                                                             \
+      }                                                                                           \
+      const int ja = wrow + hh - __builtin_amdgcn_readlane(r0l, ca);                              \
+      const int jb = wrow + hh - __builtin_amdgcn_readlane(r0l, cb & 63);                         \
+      sep_entry2<C0>(acc, bb + ca, bb + cb, vb[ca * kRegPitch + coil], vb[cb * kRegPitch + coil], \
+                     ja, jb, (unsigned)J0);                                                       \
     }                                                                                             \
   }
     TKBN_SEP_GROUP(-5) TKBN_SEP_GROUP(-4) TKBN_SEP_GROUP(-3) TKBN_SEP_GROUP(-2) TKBN_SEP_GROUP(-1)
@@ -1537,8 +1562,8 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
     TKBN_SEP_GROUP(10) TKBN_SEP_GROUP(11) TKBN_SEP_GROUP(12) TKBN_SEP_GROUP(13) TKBN_SEP_GROUP(14)
     TKBN_SEP_GROUP(15)
 #undef TKBN_SEP_GROUP
-    __syncthreads();
   }
+  lds_barrier();  // the flush reuses buffer 0
 
   // Flush: every owned cell inside the grid is written (split tiles were zeroed first
   // and add), rotated back by conj(P_0 P_1).  The rows go out through LDS in rounds of
@@ -1551,7 +1576,7 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
       const int slot = (wave % kRoundWaves) * kRegRowsPerWave + hh;
       const float2 pr = rph[kRound * round + slot];
 #pragma unroll
-      for (int c = 0; c < kRegT1; ++c) vals[(slot * kRegT1 + c) * kRegPitch + coil] = cmul(acc[c], cmul(pr, cphs[c]));
+      for (int c = 0; c < kRegT1; ++c) vals[0][(slot * kRegT1 + c) * kRegPitch + coil] = cmul(acc[c], cmul(pr, cphs[c]));
     }
     lds_barrier();
 #pragma unroll
@@ -1560,7 +1585,7 @@ __global__ void __launch_bounds__(kRegThreads) __attribute__((amdgpu_waves_per_e
       const int c = e & (kRegT1 - 1), cl = (e / kRegT1) & (kRegCoils - 1), slot = e / (kRegT1 * kRegCoils);
       const int k0 = o0 + kRound * round + slot, k1 = o1 + c;
       if (cl < ncc && k0 < g.K[0] && k1 < g.K[1]) {
-        const float2 v = vals[(slot * kRegT1 + c) * kRegPitch + cl];
+        const float2 v = vals[0][(slot * kRegT1 + c) * kRegPitch + cl];
         float2* dst = grid + (rbase + cl) * g.G + (int64_t)k0 * g.stride[0] + k1;
         if (nsplit == 1) {
           *dst = v;
