@@ -152,16 +152,17 @@ def test_nonfinite_and_empty_inputs():
     assert z.shape == (1, 2, 32, 32) and torch.count_nonzero(z) == 0
 
 
-@pytest.mark.parametrize("ndim", [1, 2, 3])
-def test_interp_domega_matches_exact_kb_fd(ndim):
+@pytest.mark.parametrize("ndim,rows", [(1, 2), (2, 2), (3, 2), (2, 5), (3, 5)])
+def test_interp_domega_matches_exact_kb_fd(ndim, rows):
     """d/domega of Re<g, A(omega) x>: HIP gradient vs FP64 central differences of the
-    exact-KB operator (smooth in omega; the table path itself is piecewise constant)."""
+    exact-KB operator (smooth in omega; the table path itself is piecewise constant).
+    Two rows run the per-lane kernel, five the kernel with taps shared through LDS."""
     im = {1: (40,), 2: (24, 20), 3: (10, 12, 8)}[ndim]
     gen = torch.Generator().manual_seed(10 + ndim)
     grid_size = tuple(2 * v for v in im)
     M = 300
-    x = gi.cplx((1, 2) + grid_size, gen, torch.complex128)
-    gout = gi.cplx((1, 2, M), gen, torch.complex128)
+    x = gi.cplx((1, rows) + grid_size, gen, torch.complex128)
+    gout = gi.cplx((1, rows, M), gen, torch.complex128)
     om = create_ktraj(ndim, M, gen)
     fw = tkbn.KbInterp(im_size=im, dtype=torch.complex128).to(DEV)
     omd = om.to(DEV).requires_grad_(True)
@@ -184,6 +185,33 @@ def test_interp_domega_matches_exact_kb_fd(ndim):
     err = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
     print(f"[domega {ndim}d] rel err vs exact-KB FD: {err:.2e}")
     assert err < 5e-3
+
+
+@pytest.mark.parametrize("ndim,J,dtype", [(2, 6, torch.complex64), (2, 5, torch.complex128),
+                                           (3, 6, torch.complex64), (3, 4, torch.complex128)])
+def test_domega_shared_taps_equal_sum_of_rows(ndim, J, dtype):
+    """The trajectory gradient of 6 rows (taps shared through LDS, lanes over rows) equals
+    the sum of the 6 one-row gradients (per-lane kernel) to rounding, with every
+    numpoints at the compile-time maximum (unguarded taps) and below it (guarded)."""
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    im = {2: (30, 26), 3: (12, 10, 14)}[ndim]
+    gen = torch.Generator().manual_seed(40 + ndim + J)
+    grid_size = tuple(2 * v for v in im)
+    M, rows = 700, 6
+    print(f"[case] domega shared taps {ndim}-D J {J} {dtype}", flush=True)
+    x = gi.cplx((1, rows) + grid_size, gen, dtype).to(DEV)
+    gout = gi.cplx((1, rows, M), gen, dtype).to(DEV)
+    real = torch.float32 if dtype == torch.complex64 else torch.float64
+    om = create_ktraj(ndim, M, gen).to(real).to(DEV)
+    fw = tkbn.KbInterp(im_size=im, numpoints=J, dtype=dtype).to(DEV)
+    args = (fw._tables(), fw.n_shift, fw.numpoints, fw.table_oversamp)
+    together = _interp.table_interp_domega(x, gout, om, *args)
+    apart = sum(_interp.table_interp_domega(x[:, c:c + 1].contiguous(), gout[:, c:c + 1].contiguous(), om, *args)
+                for c in range(rows))
+    err = float(torch.linalg.norm(together - apart) / torch.linalg.norm(apart))
+    print(f"[domega shared {ndim}d J{J}] rel diff vs sum of rows: {err:.2e}")
+    assert err < (1e-5 if dtype == torch.complex64 else 1e-12)
 
 
 @pytest.mark.parametrize("im_size,grid,J", [((33, 35), (65, 70), 6), ((4, 4), (4, 4), 6), ((40,), (77,), 5), ((9, 10, 7), (17, 21, 15), (4, 6, 5))])

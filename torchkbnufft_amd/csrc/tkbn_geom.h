@@ -111,48 +111,74 @@ __device__ __forceinline__ T sample_theta(const Geom<T>& g, const T* om, int64_t
   return theta;
 }
 
-template <typename T, int JM>
-__device__ __forceinline__ bool sample_taps_d(const Geom<T>& g, int d, T omega, cplx<T>* w, cplx<T>* dw,
-                                              int64_t* off) {
+// Per-dimension part of a sample's taps: tm = omega / gam, whether tm is usable, the
+// unwrapped base cell and its wrapped index.
+template <typename T>
+struct TapBase {
+  T tm;
+  bool ok;
+  int64_t base, bm;
+};
+
+template <typename T>
+__device__ __forceinline__ TapBase<T> tap_base(const Geom<T>& g, int d, T omega) {
 #pragma clang fp contract(off)
-  const T tm = omega / g.gam[d];
-  const bool ok = (fabs(tm) < (T)1.0e15);
-  const T fb = floor(tm - g.halfJ[d]);
-  const int64_t base = ok ? (int64_t)fb + 1 : 0;
+  TapBase<T> tb;
+  tb.tm = omega / g.gam[d];
+  tb.ok = (fabs(tb.tm) < (T)1.0e15);
+  const T fb = floor(tb.tm - g.halfJ[d]);
+  tb.base = tb.ok ? (int64_t)fb + 1 : 0;
   const int64_t K = g.K[d];
-  int64_t bm = base % K;
-  if (bm < 0) bm += K;
+  tb.bm = tb.base % K;
+  if (tb.bm < 0) tb.bm += K;
+  return tb;
+}
+
+// Tap j < J of dimension d: the table entry w, the derivative dw of the quadratic
+// through the entries around round(u L) at the exact u, and the flattened offset.
+template <typename T>
+__device__ __forceinline__ void tap_weight_d(const Geom<T>& g, int d, const TapBase<T>& tb, int j, cplx<T>& w,
+                                             cplx<T>& dw, int64_t& off) {
+#pragma clang fp contract(off)
   const cplx<T>* tab = g.table[d];
   const T nan = (T)NAN;
   const int tmax = g.tmax[d];
+  const int64_t gi = tb.base + j;
+  const T u = tb.tm - (T)gi;
+  const T ul = u * g.Lf[d];
+  const T r = rint(ul);
+  const T fr = ul - r;
+  int ti = (int)r + g.center[d];
+  ti = ti < 0 ? 0 : (ti > tmax ? tmax : ti);
+  const cplx<T> t0 = tab[ti];
+  const cplx<T> tm1 = ti > 0 ? tab[ti - 1] : make_c<T>(0, 0);
+  const cplx<T> tp1 = ti < tmax ? tab[ti + 1] : make_c<T>(0, 0);
+  cplx<T> dt;
+  dt.x = ((tp1.x - tm1.x) * (T)0.5 + fr * (tp1.x - (T)2 * t0.x + tm1.x)) * g.Lf[d];
+  dt.y = ((tp1.y - tm1.y) * (T)0.5 + fr * (tp1.y - (T)2 * t0.y + tm1.y)) * g.Lf[d];
+  w = tb.ok ? t0 : make_c<T>(nan, nan);
+  dw = tb.ok ? dt : make_c<T>(nan, nan);
+  int64_t md = tb.bm + j;
+  const int64_t K = g.K[d];
+  while (md >= K) md -= K;
+  off = md * g.stride[d];
+}
+
+template <typename T, int JM>
+__device__ __forceinline__ bool sample_taps_d(const Geom<T>& g, int d, T omega, cplx<T>* w, cplx<T>* dw,
+                                              int64_t* off) {
+  const TapBase<T> tb = tap_base<T>(g, d, omega);
 #pragma unroll
   for (int j = 0; j < JM; ++j) {
     if (j < g.J[d]) {
-      const int64_t gi = base + j;
-      const T u = tm - (T)gi;
-      const T ul = u * g.Lf[d];
-      const T r = rint(ul);
-      const T fr = ul - r;
-      int ti = (int)r + g.center[d];
-      ti = ti < 0 ? 0 : (ti > tmax ? tmax : ti);
-      const cplx<T> t0 = tab[ti];
-      const cplx<T> tm1 = ti > 0 ? tab[ti - 1] : make_c<T>(0, 0);
-      const cplx<T> tp1 = ti < tmax ? tab[ti + 1] : make_c<T>(0, 0);
-      cplx<T> dt;
-      dt.x = ((tp1.x - tm1.x) * (T)0.5 + fr * (tp1.x - (T)2 * t0.x + tm1.x)) * g.Lf[d];
-      dt.y = ((tp1.y - tm1.y) * (T)0.5 + fr * (tp1.y - (T)2 * t0.y + tm1.y)) * g.Lf[d];
-      w[j] = ok ? t0 : make_c<T>(nan, nan);
-      dw[j] = ok ? dt : make_c<T>(nan, nan);
-      int64_t md = bm + j;
-      while (md >= K) md -= K;
-      off[j] = md * g.stride[d];
+      tap_weight_d<T>(g, d, tb, j, w[j], dw[j], off[j]);
     } else {
       w[j] = make_c<T>(0, 0);
       dw[j] = make_c<T>(0, 0);
       off[j] = 0;
     }
   }
-  return ok;
+  return tb.ok;
 }
 
 inline int validate_desc(const tkbn_interp_desc* d) {

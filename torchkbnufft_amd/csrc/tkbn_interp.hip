@@ -241,6 +241,146 @@ __global__ void __launch_bounds__(256)
   for (int d = 0; d < D; ++d) o[d * g.M + m] = acc[d];
 }
 
+// Trajectory gradient with the taps shared through LDS (2-D / 3-D, numpoints <= 6, at
+// least 4 rows): a block of 512 threads holds P = 512 / S samples with S = 4 or 8 row
+// slots each; wave w serves 64 consecutive samples of one slot, so its gathers of a tap
+// meet neighbouring samples of one row (coalesced as in the per-lane kernel).  The
+// threads of a sample first compute its D x 6 taps together into LDS, then each sums
+// its rows (slot, slot + S, ...) with the taps read from LDS (the plane of 36 (3-D) or
+// all 36 (2-D) gathers issued before their use), and the slots' partial sums are added
+// in slot order.  Compared with one lane per sample holding all taps and rows in
+// registers, this keeps ~70 registers free and S times more waves in flight.
+constexpr int kDomJ = 6;
+constexpr int kDomMinS = 4;
+constexpr int kDomThreads = 512;
+
+template <typename T, int D, bool FULL>
+__global__ void __launch_bounds__(kDomThreads)
+    interp_domega_shared_kernel(const Geom<T> g, const cplx<T>* __restrict__ grid, const cplx<T>* __restrict__ gout,
+                                const T* __restrict__ omega, T* __restrict__ out, int S) {
+  using C2 = cplx<T>;
+  constexpr int kMaxP = kDomThreads / kDomMinS;
+  __shared__ C2 sw[D][kDomJ][kMaxP], sdw[D][kDomJ][kMaxP];
+  __shared__ int soff[D][kDomJ][kMaxP];
+  __shared__ T sth[kMaxP];
+  __shared__ T red[D][kDomThreads];
+  const int P = kDomThreads / S;
+  const int slot = (int)threadIdx.x / P, p = (int)threadIdx.x - slot * P;
+  const int64_t m = (int64_t)blockIdx.x * P + p;
+  const bool live = m < g.M;  // dead samples compute taps of omega 0; their sums are dropped
+  const int64_t b = blockIdx.z;
+  const T* om = omega + (g.batched ? b * D * g.M : 0);
+  for (int t = slot; t < D * kDomJ; t += S) {
+    const int d = t / kDomJ, j = t - d * kDomJ;
+    C2 w = make_c<T>(0, 0), dw = make_c<T>(0, 0);
+    int64_t off = 0;
+    if (j < g.J[d]) tap_weight_d<T>(g, d, tap_base<T>(g, d, live ? om[d * g.M + m] : (T)0), j, w, dw, off);
+    sw[d][j][p] = w;
+    sdw[d][j][p] = dw;
+    soff[d][j][p] = (int)off;  // G < 2^31 (host)
+  }
+  if (slot == 0) sth[p] = live ? sample_theta<T, D>(g, om, m) : (T)0;
+  __syncthreads();
+  const T theta = sth[p];
+  const C2 phase = make_c<T>(cos(theta), sin(theta));
+  const C2 zero = make_c<T>(0, 0);
+
+  T acc[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) acc[d] = 0;
+  const int64_t nrows = g.batched ? g.C : g.B * g.C;
+  const int64_t rowbase = g.batched ? b * g.C : 0;
+  for (int64_t r = slot; r < nrows; r += S) {
+    const C2* gp = grid + (rowbase + r) * g.G;
+    C2 S0 = zero, Sd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) Sd[d] = zero;
+    if constexpr (D == 2) {
+      // all 36 gathers of the row are issued before any of them is used
+      C2 X[kDomJ][kDomJ];
+#pragma unroll
+      for (int j0 = 0; j0 < kDomJ; ++j0) {
+        const C2* rp = gp + soff[0][j0][p];
+#pragma unroll
+        for (int j1 = 0; j1 < kDomJ; ++j1)
+          X[j0][j1] = (FULL || (j0 < g.J[0] && j1 < g.J[1])) ? rp[soff[1][j1][p]] : zero;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j0 = 0; j0 < kDomJ; ++j0) {
+        if (FULL || j0 < g.J[0]) {
+          C2 Pw = zero, P1 = zero;
+#pragma unroll
+          for (int j1 = 0; j1 < kDomJ; ++j1) {
+            cmac_fma(Pw, sw[1][j1][p], X[j0][j1]);
+            cmac_fma(P1, sdw[1][j1][p], X[j0][j1]);
+          }
+          const C2 w0 = sw[0][j0][p], dw0 = sdw[0][j0][p];
+          cmac_fma(S0, w0, Pw);
+          cmac_fma(Sd[0], dw0, Pw);
+          cmac_fma(Sd[1], w0, P1);
+        }
+      }
+    } else {
+#pragma unroll 1
+      for (int j0 = 0; j0 < kDomJ; ++j0) {
+        if (FULL || j0 < g.J[0]) {
+          // the plane's 36 gathers are issued before any of them is used
+          C2 X[kDomJ][kDomJ];
+#pragma unroll
+          for (int j1 = 0; j1 < kDomJ; ++j1) {
+            const C2* rp = gp + soff[0][j0][p] + soff[1][j1][p];
+#pragma unroll
+            for (int j2 = 0; j2 < kDomJ; ++j2)
+              X[j1][j2] = (FULL || (j1 < g.J[1] && j2 < g.J[2])) ? rp[soff[2][j2][p]] : zero;
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          C2 Q = zero, Q1 = zero, Q2 = zero;
+#pragma unroll
+          for (int j1 = 0; j1 < kDomJ; ++j1) {
+            if (FULL || j1 < g.J[1]) {
+              C2 Pw = zero, P2 = zero;
+#pragma unroll
+              for (int j2 = 0; j2 < kDomJ; ++j2) {
+                cmac_fma(Pw, sw[2][j2][p], X[j1][j2]);
+                cmac_fma(P2, sdw[2][j2][p], X[j1][j2]);
+              }
+              const C2 w1 = sw[1][j1][p];
+              cmac_fma(Q, w1, Pw);
+              cmac_fma(Q1, sdw[1][j1][p], Pw);
+              cmac_fma(Q2, w1, P2);
+            }
+          }
+          const C2 w0 = sw[0][j0][p];
+          cmac_fma(S0, w0, Q);
+          cmac_fma(Sd[0], sdw[0][j0][p], Q);
+          cmac_fma(Sd[1], w0, Q1);
+          cmac_fma(Sd[2], w0, Q2);
+        }
+      }
+    }
+    // q = conj(G) * phase;  grad_d += Re(q * (i s_d S0 + S_d / gam_d))
+    const C2 q = cmul_conj(phase, gout[(rowbase + r) * g.M + (live ? m : 0)]);
+    const C2 qs = cmul(q, S0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const C2 qd = cmul(q, Sd[d]);
+      acc[d] += qd.x / g.gam[d] - g.nshift[d] * qs.y;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) red[d][threadIdx.x] = acc[d];
+  __syncthreads();
+  if (slot != 0 || !live) return;
+  T* o = out + (g.batched ? b * D * g.M : 0);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    T sum = red[d][p];
+    for (int k = 1; k < S; ++k) sum += red[d][k * P + p];
+    o[d * g.M + m] = sum;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -270,6 +410,26 @@ template <typename T, int D, int JM>
 int domega_launch(const Geom<T>& g, const void* grid, const void* gout, const void* omega, void* out, hipStream_t s) {
   const int64_t zdim = g.batched ? g.B : 1;
   const int64_t nrows = g.batched ? g.C : g.B * g.C;
+  if constexpr (D >= 2 && JM == kDomJ) {
+    if (nrows >= kDomMinS && g.G < (1LL << 31)) {
+      const int S = nrows > kDomMinS ? 8 : kDomMinS;
+      const int P = kDomThreads / S;
+      dim3 grd((unsigned)ceil_div(g.M, P), 1, (unsigned)zdim);
+      bool full = true;
+      for (int d = 0; d < D; ++d) full = full && g.J[d] == kDomJ;
+      if (full)
+        interp_domega_shared_kernel<T, D, true><<<grd, dim3(kDomThreads), 0, s>>>(
+            g, static_cast<const cplx<T>*>(grid), static_cast<const cplx<T>*>(gout), static_cast<const T*>(omega),
+            static_cast<T*>(out), S);
+      else
+        interp_domega_shared_kernel<T, D, false><<<grd, dim3(kDomThreads), 0, s>>>(
+            g, static_cast<const cplx<T>*>(grid), static_cast<const cplx<T>*>(gout), static_cast<const T*>(omega),
+            static_cast<T*>(out), S);
+      TKBN_LAUNCH_CHECK("interp_domega_shared_kernel");
+      note_kernel("interp_domega_shared_kernel");
+      return TKBN_OK;
+    }
+  }
   // lanes per sample, so that more rows' gathers are in flight together; each lane
   // recomputes the sample's taps, which in 3-D (3 x 6 table pairs) costs more than the
   // parallel rows gain (kooshball gradient 16.4 -> 19.4 ms with 8 lanes): one lane there
