@@ -672,8 +672,9 @@ def bench_kooshball(dev, steps: int, warmup: int) -> dict:
 
 
 def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs (profiles/traffic_sense2d_radial_640.*, calibrated on a plain copy
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 runs of the TCC/EA
+    read and write request counters summed over every TCC instance
+    (profiles/traffic_sense2d_radial_640.*; the request sizes are checked on a plain copy
     kernel); None when that profile is absent or does not cover the kernel."""
     path = os.path.join(ROOT, "profiles", "traffic_sense2d_radial_640.json")
     try:
@@ -709,7 +710,7 @@ def main() -> int:
                     help="sense2d: the headline SENSE pair (default); kooshball3d: the 3-D deterministic case; "
                     "batched: the 64 uneven spiral problems only (fused batch, separate launches, plain loop); "
                     "grad: the image + trajectory gradient form only; "
-                    "copy512: a plain 512 MiB read + 512 MiB write kernel (calibrates rocprofv3 FETCH/WRITE_SIZE); "
+                    "copy512: a plain 512 MiB read + 512 MiB write kernel (checks the request sizes of the traffic counters); "
                     "spmat: the SENSE batch with sparse-matrix interpolation only")
     args = ap.parse_args()
     if args.gpus != 1:
