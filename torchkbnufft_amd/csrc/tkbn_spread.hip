@@ -1246,6 +1246,29 @@ __global__ void __launch_bounds__(1024) sep_plan_scan_kernel(const TileGeom tg, 
 // with real w0, w1; exception entries use complex weights t_d * conj(P_d).
 __device__ __forceinline__ float2 sep_gat_real(const float (&w0)[6], const float (&w1)[6], const float2* L0, int J0,
                                                int J1, bool fixed6) {
+  if (fixed6) {
+    // J = 6: all 36 taps are read before any is used (a scheduling barrier keeps the
+    // sums behind the reads, so the LDS latency is paid once per entry)
+    float2 X[6][6];
+#pragma unroll
+    for (int j0 = 0; j0 < 6; ++j0)
+#pragma unroll
+      for (int j1 = 0; j1 < 6; ++j1) X[j0][j1] = L0[j0 * (kGatR1 * kGatPitch) + j1 * kGatPitch];
+    __builtin_amdgcn_sched_barrier(0);
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j0 = 0; j0 < 6; ++j0) {
+      float2 p = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j1 = 0; j1 < 6; ++j1) {
+        p.x = fmaf(w1[j1], X[j0][j1].x, p.x);
+        p.y = fmaf(w1[j1], X[j0][j1].y, p.y);
+      }
+      acc.x = fmaf(w0[j0], p.x, acc.x);
+      acc.y = fmaf(w0[j0], p.y, acc.y);
+    }
+    return acc;
+  }
   float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
   for (int j0 = 0; j0 < 6; ++j0) {
