@@ -598,6 +598,31 @@ def bench_small(dev, iters: int, with_cpu: bool) -> dict:
         "gpu_pairs_per_s": 1e3 / ms,
         "fp64_oracle_rel_err": nudft_rel_err(fw(xd, od)[0, 0, sel], xd[0, 0], od[:, sel], (128, 128)),
     }
+    # The same pair captured once as a HIP graph and replayed: the eager pair is bound by
+    # host launch work (17 launches and the modules' Python), the replay by the kernels.
+    try:
+        gs = torch.cuda.Stream()
+        gs.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(gs):
+            for _ in range(3):
+                ad(fw(xd, od), od)  # plans and workspaces of the capture stream
+        torch.cuda.current_stream().wait_stream(gs)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=gs):
+            yg = ad(fw(xd, od), od)
+        diffs = []
+        for _ in range(3):  # every replay against the eager pair, eager work in between
+            graph.replay()
+            torch.cuda.synchronize()
+            ye = ad(fw(xd, od), od)
+            diffs.append(float(torch.linalg.norm(yg - ye) / torch.linalg.norm(ye)))
+        gms = timed(graph.replay, iters)
+        out["graph_ms"] = gms
+        out["graph_pairs_per_s"] = 1e3 / gms
+        out["graph_rel_diff_max"] = max(diffs)
+    except RuntimeError as e:  # reported, not fatal: the eager figures stand
+        out["graph_error"] = str(e)[:200]
     if with_cpu:
         from oracle import table_path as tp
 

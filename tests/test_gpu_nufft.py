@@ -214,3 +214,33 @@ def test_ktraj_gradient_matches_nudft_derivative():
     err = np.linalg.norm(grad - ref) / np.linalg.norm(ref)
     print(f"[ktraj grad] rel err vs FP64 NUDFT derivative: {err:.2e}")
     assert err < 1e-2
+
+
+@pytest.mark.parametrize("coils", [1, 40])
+def test_pair_graph_replay_matches_eager(coils):
+    """A forward + adjoint pair captured as a HIP graph (after warm-up calls on the
+    capture stream build the plans) replays to the eager result, replay after replay,
+    with eager work in between: the library's per-call work (plan-cache check, early-exit
+    plan kernels, fill kernels) is capturable.  40 coil rows run the multi-coil kernels."""
+    print(f"[case] graph replay, {coils} coil rows", flush=True)
+    gen = torch.Generator().manual_seed(61)
+    im = (48, 40)
+    x = gi.cplx((1, coils) + im, gen, torch.complex64).to(DEV)
+    om = create_ktraj(2, 3000, gen).to(DEV)
+    fw = tkbn.KbNufft(im_size=im).to(DEV)
+    ad = tkbn.KbNufftAdjoint(im_size=im).to(DEV)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(3):
+            ad(fw(x, om), om)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        y = ad(fw(x, om), om)
+    for _ in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        ref = ad(fw(x, om), om)
+        assert rel_l2(y, ref) < 1e-6
