@@ -17,6 +17,7 @@ import torch
 from torch import Tensor
 
 from .. import _lib
+from .._autograd._apply import run as _run
 from ._host import host_values, require_gpu
 
 
@@ -115,8 +116,8 @@ def fft_and_scale(image: Tensor, scaling_coef: Tensor, im_size: Tensor, grid_siz
     im = host_values(im_size)
     scale = _norm_scale(norm, gs)
     if fused_supported(image.dtype, im, gs):
-        return GridForward.apply(image, None, scaling_coef, im, gs, scale)
-    grid = ApodPad.apply(image, None, scaling_coef, im, gs, scale)
+        return _run(GridForward, image, None, scaling_coef, im, gs, scale)
+    grid = _run(ApodPad, image, None, scaling_coef, im, gs, scale)
     return torch.fft.fftn(grid, dim=tuple(range(-len(gs), 0)), norm="backward")
 
 
@@ -128,9 +129,9 @@ def ifft_and_scale(image: Tensor, scaling_coef: Tensor, im_size: Tensor, grid_si
     im = host_values(im_size)
     scale = _norm_scale(norm, gs)
     if fused_supported(image.dtype, im, gs):
-        return GridAdjoint.apply(image, None, scaling_coef, im, scale)
+        return _run(GridAdjoint, image, None, scaling_coef, im, scale)
     x = torch.fft.ifftn(image, dim=tuple(range(-len(gs), 0)), norm="forward")
-    return CropDeapod.apply(x, None, scaling_coef, im, scale)
+    return _run(CropDeapod, x, None, scaling_coef, im, scale)
 
 
 # ---------------------------------------------------------------------------
@@ -315,11 +316,11 @@ def toeplitz_filter(image: Tensor, smaps: Optional[Tensor], kernel: Tensor, norm
         and not (torch.is_grad_enabled() and (kernel.requires_grad or (smaps is not None and smaps.requires_grad)))
     )
     if one_pass:
-        return FftFilter.apply(image, smaps, kernel, lead != (), scale, scale)
+        return _run(FftFilter, image, smaps, kernel, lead != (), scale, scale)
     if fused:
         ones = _ones_scaling(im, image.dtype, image.device)
-        grid = GridForward.apply(image, smaps, ones, im, gs, scale)
-        return GridAdjoint.apply(grid * kernel, smaps, ones, im, scale)
+        grid = _run(GridForward, image, smaps, ones, im, gs, scale)
+        return _run(GridAdjoint, grid * kernel, smaps, ones, im, scale)
     x = image * smaps if smaps is not None else image
     pad = []
     for g, n in zip(reversed(gs), reversed(im)):

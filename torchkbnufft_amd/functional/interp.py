@@ -6,6 +6,7 @@ from typing import List, Tuple
 import torch
 from torch import Tensor
 
+from .._autograd._apply import run as _run
 from .._autograd.interp import (
     KbSpmatInterpAdjoint,
     KbSpmatInterpForward,
@@ -33,7 +34,7 @@ def kb_table_interp(
             raise ValueError("For real inputs, last dimension must be size 2.")
         is_complex = False
         image = torch.view_as_complex(image.contiguous())
-    data = KbTableInterpForward.apply(image, omega, tables, n_shift, numpoints, table_oversamp, offsets)
+    data = _run(KbTableInterpForward, image, omega, tables, n_shift, numpoints, table_oversamp, offsets)
     if not is_complex:
         data = torch.view_as_real(data)
     return data
@@ -56,7 +57,7 @@ def kb_table_interp_adjoint(
             raise ValueError("For real inputs, last dimension must be size 2.")
         is_complex = False
         data = torch.view_as_complex(data.contiguous())
-    image = KbTableInterpAdjoint.apply(data, omega, tables, n_shift, numpoints, table_oversamp, offsets, grid_size)
+    image = _run(KbTableInterpAdjoint, data, omega, tables, n_shift, numpoints, table_oversamp, offsets, grid_size)
     if not is_complex:
         image = torch.view_as_real(image)
     return image
@@ -73,7 +74,7 @@ def kb_spmat_interp(image: Tensor, interp_mats: Tuple[Tensor, Tensor]) -> Tensor
             raise ValueError("For real inputs, last dimension must be size 2.")
         is_complex = False
         image = torch.view_as_complex(image.contiguous())
-    data = KbSpmatInterpForward.apply(image, interp_mats)
+    data = _run(KbSpmatInterpForward, image, interp_mats)
     if not is_complex:
         data = torch.view_as_real(data)
     return data
@@ -87,7 +88,7 @@ def kb_spmat_interp_adjoint(data: Tensor, interp_mats: Tuple[Tensor, Tensor], gr
             raise ValueError("For real inputs, last dimension must be size 2.")
         is_complex = False
         data = torch.view_as_complex(data.contiguous())
-    image = KbSpmatInterpAdjoint.apply(data, interp_mats, grid_size)
+    image = _run(KbSpmatInterpAdjoint, data, interp_mats, grid_size)
     if not is_complex:
         image = torch.view_as_real(image)
     return image

@@ -6,6 +6,7 @@ from typing import List, Optional, Tuple
 import torch
 from torch import Tensor
 
+from .._autograd._apply import run as _run
 from .._autograd.fft import ApodPad, CropDeapod, GridAdjoint, GridForward
 from .._nufft._host import host_values
 from .._nufft.fft import _norm_scale, fused_supported
@@ -16,8 +17,8 @@ def _to_grid(image, smaps, scaling_coef, im_size, grid_size, norm):
     gs = host_values(grid_size)
     im = host_values(im_size)
     if fused_supported(image.dtype, im, gs):
-        return GridForward.apply(image, smaps, scaling_coef, im, gs, _norm_scale(norm, gs))
-    grid = ApodPad.apply(image, smaps, scaling_coef, im, gs, _norm_scale(norm, gs))
+        return _run(GridForward, image, smaps, scaling_coef, im, gs, _norm_scale(norm, gs))
+    grid = _run(ApodPad, image, smaps, scaling_coef, im, gs, _norm_scale(norm, gs))
     return torch.fft.fftn(grid, dim=tuple(range(-len(gs), 0)), norm="backward")
 
 
@@ -25,9 +26,9 @@ def _from_grid(grid, smaps, scaling_coef, im_size, grid_size, norm):
     gs = host_values(grid_size)
     im = host_values(im_size)
     if fused_supported(grid.dtype, im, gs):
-        return GridAdjoint.apply(grid, smaps, scaling_coef, im, _norm_scale(norm, gs))
+        return _run(GridAdjoint, grid, smaps, scaling_coef, im, _norm_scale(norm, gs))
     grid = torch.fft.ifftn(grid, dim=tuple(range(-len(gs), 0)), norm="forward")
-    return CropDeapod.apply(grid, smaps, scaling_coef, im, _norm_scale(norm, gs))
+    return _run(CropDeapod, grid, smaps, scaling_coef, im, _norm_scale(norm, gs))
 
 
 def _nufft_forward(image, smaps, scaling_coef, im_size, grid_size, omega, tables, n_shift, numpoints,
