@@ -315,16 +315,17 @@ struct uint_of<double> {
 
 // max |re|, |im| over kdata, or +inf when kdata or omega holds a non-finite value
 // (fixed-point accumulation cannot carry those; the float-atomic kernel then runs)
+// Block `blk` of `nblk` (256 threads) of the max-|kdata| pass.
 template <typename T>
-__global__ void __launch_bounds__(256) maxabs_kernel(const T* __restrict__ x, int64_t n, const T* __restrict__ om,
-                                                     int64_t nom, void* slot) {
+__device__ __forceinline__ void maxabs_block(const T* __restrict__ x, int64_t n, const T* __restrict__ om, int64_t nom,
+                                             void* slot, int64_t blk, int64_t nblk) {
   using U = typename uint_of<T>::type;
   T mx = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = blk * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
     const T v = fabs(x[i]);
     mx = isfinite(v) ? (v > mx ? v : mx) : (T)INFINITY;
   }
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nom; i += (int64_t)gridDim.x * blockDim.x)
+  for (int64_t i = blk * blockDim.x + threadIdx.x; i < nom; i += nblk * blockDim.x)
     if (!isfinite(om[i])) mx = (T)INFINITY;
   // wave reduce, block reduce in LDS, then one atomic per block (few blocks: the
   // atomics all hit one address)
@@ -341,6 +342,12 @@ __global__ void __launch_bounds__(256) maxabs_kernel(const T* __restrict__ x, in
     memcpy(&bits, &mx, sizeof(T));
     atomicMax(static_cast<U*>(slot), bits);  // non-negative floats order like their bits
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) maxabs_kernel(const T* __restrict__ x, int64_t n, const T* __restrict__ om,
+                                                     int64_t nom, void* slot) {
+  maxabs_block<T>(x, n, om, nom, slot, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -376,6 +383,24 @@ template <typename T, int D>
 __global__ void __launch_bounds__(256)
     zero_hot_tiles_kernel(const Geom<T> g, const TileGeom tg, const int* __restrict__ ws, cplx<T>* __restrict__ grid) {
   zero_hot_tiles_item<T, D>(g, tg, ws, grid, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// The zero pass of split tiles and the max-|kdata| pass as one launch (both must finish
+// before the spread; one launch saves the host a few microseconds per call): blocks
+// [0, NB * ychunks * zdim) zero, the last `nmax` blocks take the max.
+template <typename T, int D>
+__global__ void __launch_bounds__(256)
+    zero_hot_maxabs_kernel(const Geom<T> g, const TileGeom tg, const int* __restrict__ ws, cplx<T>* __restrict__ grid,
+                           int64_t ychunks, const T* __restrict__ x, int64_t n, const T* __restrict__ om, int64_t nom,
+                           void* slot, int64_t nmax) {
+  const int64_t nzero = (int64_t)tg.NB * ychunks * (g.batched ? g.B : 1);
+  const int64_t blk = blockIdx.x;
+  if (blk >= nzero) {
+    maxabs_block<T>(x, n, om, nom, slot, blk - nzero, nmax);
+    return;
+  }
+  const int64_t t = blk % tg.NB, rest = blk / tg.NB;
+  zero_hot_tiles_item<T, D>(g, tg, ws, grid, (int)t, rest % ychunks, rest / ychunks);
 }
 
 // ---------------------------------------------------------------------------
@@ -585,6 +610,21 @@ __global__ void __launch_bounds__(spread_tiles_threads(D))
   // alignment, and the static LDS above this array is not a multiple of 8 bytes
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   spread_tiles_item<T, FIXED, D, JM>(g, tg, kdata, omega, ws, maxabs_slot, grid, fallback, blockIdx.x, blockIdx.y,
+                                     blockIdx.z, smem_raw);
+}
+
+// Fixed point, or (when the max slot says kdata or omega holds a non-finite value) the
+// float-atomic pass, in one launch: the two items return at once unless the slot selects
+// them (the LDS is sized for the fixed-point accumulators, the larger of the two).
+template <typename T, int D, int JM>
+__global__ void __launch_bounds__(spread_tiles_threads(D))
+    spread_tiles_either_kernel(const Geom<T> g, const TileGeom tg, const cplx<T>* __restrict__ kdata,
+                               const T* __restrict__ omega, const int* __restrict__ ws,
+                               const void* __restrict__ maxabs_slot, cplx<T>* __restrict__ grid) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  spread_tiles_item<T, true, D, JM>(g, tg, kdata, omega, ws, maxabs_slot, grid, 0, blockIdx.x, blockIdx.y, blockIdx.z,
+                                    smem_raw);
+  spread_tiles_item<T, false, D, JM>(g, tg, kdata, omega, ws, maxabs_slot, grid, 1, blockIdx.x, blockIdx.y,
                                      blockIdx.z, smem_raw);
 }
 
@@ -1839,16 +1879,11 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
   }
   const unsigned ychunks = (unsigned)ceil_div(nrows, tg.CC);
   const dim3 grid3((unsigned)tg.maxwork, ychunks, (unsigned)zdim);
-  if (!det) {
-    zero_hot_tiles_kernel<T, D><<<dim3((unsigned)tg.NB, ychunks, (unsigned)zdim), dim3(256), 0, s>>>(
-        g, tg, ws, static_cast<cplx<T>*>(grid));
-    TKBN_LAUNCH_CHECK("zero_hot_tiles_kernel");
-  }
   const auto* kd = static_cast<const cplx<T>*>(kdata);
   const auto* om = static_cast<const T*>(omega);
   auto* gr = static_cast<cplx<T>*>(grid);
   if (det || std::is_same<T, float>::value) {
-    // fixed point; the float kernel only runs if kdata or omega is not finite.  The
+    // fixed point; the float pass only runs if kdata or omega is not finite.  The
     // slot starts at zero: cleared by the plan-cache check, or here without one
     if (!cache) {
       const int st = launch_zero_fill<unsigned long long>(static_cast<unsigned long long*>(maxabs_slot), 1, s);
@@ -1857,15 +1892,25 @@ int spread_launch(const Geom<T>& g, const TileGeom& tg, bool det, const void* kd
     const int64_t n = g.B * g.C * g.M * 2;
     const int64_t nom = (g.batched ? g.B : 1) * D * g.M;
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(std::max(n, nom), 1024), 256));
-    maxabs_kernel<T><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(static_cast<const T*>(kdata), n, om, nom,
-                                                                   maxabs_slot);
-    TKBN_LAUNCH_CHECK("maxabs_kernel");
-    spread_tiles_kernel<T, true, D, JM><<<grid3, dim3(spread_tiles_threads(D)), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 0);
-    TKBN_LAUNCH_CHECK("spread_tiles_kernel<fixed>");
-    spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(spread_tiles_threads(D)), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 1);
-    TKBN_LAUNCH_CHECK("spread_tiles_kernel<non-finite>");
+    if (!det) {
+      const int64_t nzero = (int64_t)tg.NB * ychunks * zdim;
+      TKBN_REQUIRE(nzero + blocks < (1LL << 31), TKBN_ERR_UNSUPPORTED, "too many tiles x row chunks");
+      zero_hot_maxabs_kernel<T, D><<<dim3((unsigned)(nzero + blocks)), dim3(256), 0, s>>>(
+          g, tg, ws, gr, (int64_t)ychunks, static_cast<const T*>(kdata), n, om, nom, maxabs_slot, blocks);
+      TKBN_LAUNCH_CHECK("zero_hot_maxabs_kernel");
+    } else {
+      maxabs_kernel<T><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(static_cast<const T*>(kdata), n, om, nom,
+                                                                     maxabs_slot);
+      TKBN_LAUNCH_CHECK("maxabs_kernel");
+    }
+    spread_tiles_either_kernel<T, D, JM><<<grid3, dim3(spread_tiles_threads(D)), lds, s>>>(g, tg, kd, om, ws,
+                                                                                          maxabs_slot, gr);
+    TKBN_LAUNCH_CHECK("spread_tiles_either_kernel");
     note_kernel(det ? "spread_tiles_kernel<fixed,deterministic>" : "spread_tiles_kernel<fixed>");
   } else {
+    zero_hot_tiles_kernel<T, D><<<dim3((unsigned)tg.NB, ychunks, (unsigned)zdim), dim3(256), 0, s>>>(
+        g, tg, ws, static_cast<cplx<T>*>(grid));
+    TKBN_LAUNCH_CHECK("zero_hot_tiles_kernel");
     spread_tiles_kernel<T, false, D, JM><<<grid3, dim3(spread_tiles_threads(D)), lds, s>>>(g, tg, kd, om, ws, maxabs_slot, gr, 0);
     TKBN_LAUNCH_CHECK("spread_tiles_kernel");
     note_kernel("spread_tiles_kernel<float>");
