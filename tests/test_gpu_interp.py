@@ -214,6 +214,29 @@ def test_domega_shared_taps_equal_sum_of_rows(ndim, J, dtype):
     assert err < (1e-5 if dtype == torch.complex64 else 1e-12)
 
 
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_domega_shared_taps_batched_omega(ndim):
+    """Per-batch trajectories (omega (B, D, M)) with 5 rows each run the shared-tap
+    kernel once for all batches: equal to one call per batch item."""
+    from torchkbnufft_amd._nufft import interp as _interp
+
+    im = {2: (20, 18), 3: (8, 10, 8)}[ndim]
+    gen = torch.Generator().manual_seed(70 + ndim)
+    grid_size = tuple(2 * v for v in im)
+    B, C, M = 3, 5, 400
+    print(f"[case] domega shared taps, batched omega {ndim}-D", flush=True)
+    x = gi.cplx((B, C) + grid_size, gen, torch.complex64).to(DEV)
+    gout = gi.cplx((B, C, M), gen, torch.complex64).to(DEV)
+    om = torch.stack([create_ktraj(ndim, M, gen) for _ in range(B)]).to(torch.float32).to(DEV)
+    fw = tkbn.KbInterp(im_size=im).to(DEV)
+    args = (fw._tables(), fw.n_shift, fw.numpoints, fw.table_oversamp)
+    together = _interp.table_interp_domega(x, gout, om, *args)
+    for b in range(B):
+        one = _interp.table_interp_domega(x[b:b + 1].contiguous(), gout[b:b + 1].contiguous(), om[b], *args)
+        err = float(torch.linalg.norm(together[b] - one) / torch.linalg.norm(one))
+        assert err < 1e-6, (b, err)
+
+
 @pytest.mark.parametrize("im_size,grid,J", [((33, 35), (65, 70), 6), ((4, 4), (4, 4), 6), ((40,), (77,), 5), ((9, 10, 7), (17, 21, 15), (4, 6, 5))])
 @pytest.mark.parametrize("deterministic", [False, True])
 def test_adjoint_awkward_grids_match_oracle(im_size, grid, J, deterministic):
