@@ -519,14 +519,25 @@ __device__ __forceinline__ void spread_tiles_item(const Geom<T>& g, const TileGe
     }
     const T theta = sample_theta<T, D>(g, om, m);
     const C2 phase = make_c<T>(cos(theta), sin(theta));
+    // the next row's sample is loaded while this row's taps are added
+    const C2* kp = kdata + (rowbase + r0) * g.M + m;
+    C2 knext = ncc > 0 ? kp[0] : make_c<T>(0, 0);
     for (int c = 0; c < ncc; ++c) {
-      const C2 v = cmul_conj(kdata[(rowbase + r0 + c) * g.M + m], phase);
+      const C2 kc = knext;
+      if (c + 1 < ncc) knext = kp[(int64_t)(c + 1) * g.M];
+      C2 v = cmul_conj(kc, phase);
+      if constexpr (FIXED) {
+        // the fixed-point scale (a power of two) applied once per row: every tap
+        // product is then the scaled one exactly
+        v.x *= fscale;
+        v.y *= fscale;
+      }
       A* are = acc + (int64_t)(2 * c) * tg.tcells;
       A* aim = are + tg.tcells;
       auto add = [&](int cell, C2 x) {
         if constexpr (FIXED) {
-          atomicAdd(are + cell, to_fixed(x.x, fscale));
-          atomicAdd(aim + cell, to_fixed(x.y, fscale));
+          atomicAdd(are + cell, to_fixed(x.x, (T)1));
+          atomicAdd(aim + cell, to_fixed(x.y, (T)1));
         } else {
           atomicAdd(are + cell, x.x);
           atomicAdd(aim + cell, x.y);
