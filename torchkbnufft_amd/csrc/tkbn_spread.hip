@@ -757,14 +757,15 @@ __device__ __forceinline__ int gat_base_tile(const Geom<float>& g, const TileGeo
 
 // Work item of group b.  Consecutive groups go to different XCDs (round robin), so
 // group b runs on XCD x = b mod 8 as that XCD's k-th group, k = b / 8.  Items are
-// dealt to the XCDs in chunks of kXcdChunk neighbouring tiles (chunk c -> XCD c mod 8):
-// tiles of a chunk share halo cells through one L2, and the hot tiles at the head of
-// the list spread over all XCDs.  A bijection of [0, n) for n a multiple of
-// 8 * kXcdChunk.
-constexpr int kXcdChunk = 4;
-__device__ __forceinline__ int64_t xcd_work_item(int64_t b) {
+// dealt to the XCDs in runs of `run` consecutive items (run r -> XCD r mod 8).  The
+// gather makes a run kXcdTiles neighbouring work items times all their coil chunks:
+// the chunks of a tile share its plan records, and neighbouring tiles the cache lines
+// of their common halo columns, through one L2, while the hot tiles at the head of the
+// list spread over all XCDs.  A bijection of [0, n) for n a multiple of 8 * run.
+constexpr int kXcdTiles = 4;
+__device__ __forceinline__ int64_t xcd_work_item(int64_t b, int run) {
   const int64_t x = b % 8, k = b / 8;
-  return kXcdChunk * (x + 8 * (k / kXcdChunk)) + k % kXcdChunk;
+  return run * (x + 8 * (k / run)) + k % run;
 }
 
 // ---------------------------------------------------------------------------
@@ -1362,7 +1363,7 @@ __global__ void __launch_bounds__(kGatThreads)
   const int lane = threadIdx.x & 63;
   const int half = lane / kGatRows, row = lane % kGatRows;  // entry slot, coil row
   const int* plan = ws;
-  const int64_t item = xcd_work_item(blockIdx.x);
+  const int64_t item = xcd_work_item(blockIdx.x, kXcdTiles * nchunk);
   const int64_t wi = item / nchunk;
   const int chunk = (int)(item - wi * nchunk);
   if (wi >= plan[tg.o_nwork]) return;
@@ -2101,7 +2102,8 @@ static int gather_sep_launch(const tkbn_interp_desc* d, const void* grid, const 
     if (st != TKBN_OK) return st;
   }
   const int nchunk = (int)ceil_div(g.B * g.C, kGatRows);
-  const int64_t groups = ceil_div(tg.maxwork * nchunk, 8 * kXcdChunk) * 8 * kXcdChunk;  // see xcd_work_item
+  const int64_t run = (int64_t)kXcdTiles * nchunk;  // see xcd_work_item
+  const int64_t groups = ceil_div(tg.maxwork * nchunk, 8 * run) * 8 * run;
   TKBN_REQUIRE(groups < (1LL << 31), TKBN_ERR_UNSUPPORTED, "too many coil rows x work items for one launch");
   profile_before(s);
   if (g.J[0] == 6 && g.J[1] == 6)
