@@ -843,6 +843,8 @@ __device__ __forceinline__ cplx<T> fwd_taps(const Geom<T>& g, const TileGeom& tg
   return acc;
 }
 
+constexpr int kFwdStageU = 8;  // staging loads in flight per thread
+
 // One work item: tile piece wi, chunk yc of tg.CC coil rows, per-batch plan b (blockIdx
 // of interp_fwd_tiled_kernel, or a batch kernel's decoded item).
 template <typename T, int D, int JM>
@@ -871,9 +873,12 @@ __device__ __forceinline__ void fwd_tiled_item(const Geom<T>& g, const TileGeom&
 #pragma unroll
   for (int d = 0; d < D; ++d) origin[d] = ((t / tg.bstride[d]) % tg.nb[d]) * tg.T[d];
 
-  // stage: lds[c * rcells + cell] = grid[row c][(origin + l) mod K]
+  // stage: lds[c * rcells + cell] = grid[row c][(origin + l) mod K], kFwdStageU loads in
+  // flight per thread before their LDS stores (one load per wait would pay a memory
+  // latency per element); the loads past the end read the last element again
   const int rc = tg.rcells;
-  for (int idx = threadIdx.x; idx < ncc * rc; idx += blockDim.x) {
+  const int nstage = ncc * rc;
+  auto stage_src = [&](int idx) {
     const int c = (int)fdiv(idx, tg.frc);
     int cell = idx - c * rc;
     int64_t gi = 0;
@@ -886,7 +891,15 @@ __device__ __forceinline__ void fwd_tiled_item(const Geom<T>& g, const TileGeom&
       while (k >= g.K[d]) k -= g.K[d];
       gi += (int64_t)k * g.stride[d];
     }
-    lds[idx] = grid[(rowbase + r0 + c) * g.G + gi];
+    return grid + (rowbase + r0 + c) * g.G + gi;
+  };
+  for (int base = threadIdx.x; base < nstage; base += kFwdStageU * blockDim.x) {
+    C2 v[kFwdStageU];
+#pragma unroll
+    for (int u = 0; u < kFwdStageU; ++u) v[u] = *stage_src(min(base + u * (int)blockDim.x, nstage - 1));
+#pragma unroll
+    for (int u = 0; u < kFwdStageU; ++u)
+      if (base + u * (int)blockDim.x < nstage) lds[base + u * blockDim.x] = v[u];
   }
   __syncthreads();
 
