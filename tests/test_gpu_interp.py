@@ -280,6 +280,25 @@ def test_adjoint_awkward_grids_match_oracle(im_size, grid, J, deterministic):
     assert rel_l2(out, tp.kb_table_interp_adjoint(plan, y, om)) < C64_TOL
 
 
+@pytest.mark.parametrize("im_size,grid,J", [((33, 35), (65, 70), 6), ((4, 4), (4, 4), 6), ((40,), (77,), 5),
+                                            ((9, 10, 7), (17, 21, 15), (4, 6, 5)), ((3, 4, 3), (5, 6, 4), 6)])
+@pytest.mark.parametrize("rows", [1, 5])
+def test_forward_awkward_grids_match_oracle(im_size, grid, J, rows):
+    """Forward gather on grids that are not tile multiples and grids smaller than the
+    kernel width: the staged tile + halo region wraps around the grid once or several
+    times per dimension."""
+    print(f"[case] forward awkward grid {grid} J {J} rows {rows}", flush=True)
+    gen = torch.Generator().manual_seed(23)
+    d = len(im_size)
+    fw = tkbn.KbInterp(im_size=im_size, grid_size=grid, numpoints=J, dtype=torch.complex64).to(DEV)
+    plan = tp.Plan(im_size, grid, J, dtype=torch.complex64)
+    M = 3000
+    om = create_ktraj(d, M, gen, torch.float32)
+    x = gi.cplx((1, rows) + tuple(grid), gen)
+    out = fw(x.to(DEV), om.to(DEV)).cpu()
+    assert rel_l2(out, tp.kb_table_interp(plan, x, om)) < C64_TOL
+
+
 def test_adjoint_deterministic_is_bitwise_repeatable():
     print("[case] deterministic adjoint, 320^2 -> 640^2 radial, 8 coils", flush=True)
     gen = torch.Generator().manual_seed(22)
