@@ -739,7 +739,7 @@ constexpr int kGatRows = 16;                // coil rows per work item
 constexpr int kGatPerWave = 64 / kGatRows;  // entries a wave takes at once
 constexpr int kGatPitch = kGatRows + 1;     // float2 per staged cell (odd: staging stores spread over banks)
 constexpr int kGatThreads = 512;
-constexpr int kGatTarget = 256;             // entries per work item
+constexpr int kGatTarget = 512;             // entries per work item
 static_assert(kGatCells * kGatPitch * 8 * 2 <= 160 * 1024, "two groups per CU");
 
 __device__ __forceinline__ int gat_base_tile(const Geom<float>& g, const TileGeom& tg, const float* om, int m, int* l) {
