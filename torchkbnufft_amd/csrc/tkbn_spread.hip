@@ -663,8 +663,8 @@ constexpr int kRegKeys = kRegT1 - kRegMinC0;  // c0 values per tile: -5 .. 15
 // A tile with more entries than kRegMaxUnsplit is split into pieces of kRegHotTarget
 // entries (their groups then add into a pre-zeroed tile); the pieces are listed
 // first so the longest work starts first.  Other tiles are one work item each.
-constexpr int kRegMaxUnsplit = 1024;
-constexpr int kRegHotTarget = 384;
+constexpr int kRegMaxUnsplit = 768;
+constexpr int kRegHotTarget = 768;
 
 __host__ __device__ inline int reg_splits(int v) {
   return v <= kRegMaxUnsplit ? 1 : (v + kRegHotTarget - 1) / kRegHotTarget;
@@ -1833,8 +1833,11 @@ static TileGeom make_tile_geom(const tkbn_interp_desc* d, bool det, bool fwd = f
   if (cc > nrows) cc = (int)(nrows > 0 ? nrows : 1);
   tg.CC = cc;
   // entries per work item: hot tiles are cut into pieces of `target` (measured: 512 for
-  // the spread, against 1024 / 2048, is 10-20 % faster on a 12-coil 256^2 spiral)
-  tg.target = det ? 0x3fffffff : (d->work_target > 0 ? d->work_target : (fwd ? 1024 : 512));
+  // the spread, against 1024 / 2048, is 10-20 % faster on a 12-coil 256^2 spiral; for the
+  // 3-D gather, whose pieces each re-stage a 60 KB region per row, 2048 against 1024 is
+  // ~1.5 % faster on the 8-coil 320^3 kooshball)
+  tg.target = det ? 0x3fffffff
+                  : (d->work_target > 0 ? d->work_target : (fwd ? (D == 3 ? 2048 : 1024) : 512));
   const int64_t M = d->klength;
   // a sample reaches at most maxt_d tiles along dimension d: 1 with a single tile,
   // 2 when every tile is at least J-1 long, 3 when the last tile is shorter
